@@ -1,0 +1,97 @@
+// kernels.hpp — argument blocks and launchers of the gfx950 LBGK kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "lattice_consts.hpp"
+
+namespace aslhip {
+
+// Step-kernel flags.
+enum : unsigned {
+  STEP_F_RHO = 1u,           // write rho of computed cells
+  STEP_F_VEL = 2u,           // write u of computed cells
+  STEP_F_RAW = 4u,           // read ghost links raw (no BC resolution)
+  STEP_F_COMPRESSIBLE = 8u,  // compressible equilibrium (ASL default)
+};
+
+// One fused collide+stream pass (AA pattern) over the ghost-inclusive block.
+// The launch is a 1-D grid of 256-cell chunks of the interior planes
+// x = 1..nx (rows y = 1..ny, every z including the two ghost columns).
+struct StepArgs {
+  char *f;                    // direction k array at f + k * qs_bytes
+  long long qs_bytes;         // bytes between direction arrays
+  int sx, sy;                 // strides of the layout: Sy*Sz and Sz
+  int nx, ny, nz;             // interior size
+  int len;                    // ghost-inclusive cell count
+  int bpp;                    // 256-cell chunks per plane
+  int nblocks;                // bpp * nx
+  const unsigned char *code;  // packed cell types
+  const unsigned short *pat;  // link pattern of CODE_FLUID_BC / CODE_GHOST cells
+  const LinkRec *rec;         // link patterns
+  const unsigned char *mode;  // resolution mode of each BC slot
+  const void *val;            // [slot][Q] addend or constant, compute type
+  void *rho, *u;              // macroscopic outputs (ASL layout)
+  double w;                   // relaxation rate (exact in the compute type)
+  unsigned flags;
+};
+
+// Boundary-link kernel entries (velocity map BCs).
+struct VLink {
+  int g;              // ghost cell (linear index)
+  unsigned comp;      // directions i whose neighbour g+c_i is a computation node
+  unsigned own;       // links g -> g+c_i this BC resolves
+  int slot;           // BC slot (parameters in VLinkArgs::bc)
+};
+
+struct VLinkBC {
+  double u[3];
+  double omega[3], centre[3];
+  int has_rot;
+};
+
+struct VLinkArgs {
+  char *f;
+  long long qs_bytes;
+  int sx, sy;
+  const VLink *links;
+  int nlinks;
+  const VLinkBC *bc;  // indexed by slot
+  double dx, pos0[3]; // position of ghost-inclusive index 0 is pos0 - dx
+  int after_odd;      // state after an odd step (else after an even one / init)
+};
+
+struct InitArgs {
+  char *f;
+  long long qs_bytes;
+  int len;
+  double rho0, u0[3];       // constants (used when rho_in/u_in are null)
+  const void *rho_in, *u_in;  // fields in the macroscopic type
+  void *rho, *u;
+};
+
+struct UnpermuteArgs {
+  const char *f;
+  long long qs_bytes;
+  int sx, sy, len;
+  int after_odd;
+  void *out;  // Q * len macroscopic type
+};
+
+struct ReduceArgs {
+  const char *f;
+  long long qs_bytes;
+  int sx, sy, len;
+  int after_odd;
+  const unsigned char *code;
+  double *partial;  // 5 per block
+};
+
+hipError_t launch_step(int q, int dtype, bool odd, bool periodic, const StepArgs &a, hipStream_t s);
+hipError_t launch_vlinks(int q, int dtype, const VLinkArgs &a, hipStream_t s);
+hipError_t launch_init(int q, int dtype, const InitArgs &a, hipStream_t s);
+hipError_t launch_unpermute(int q, int dtype, const UnpermuteArgs &a, hipStream_t s);
+// partial must hold 5 * reduce_blocks(len) doubles; out: 5 doubles (device).
+int reduce_blocks(int len);
+hipError_t launch_reduce(int q, int dtype, const ReduceArgs &a, double *out, hipStream_t s);
+
+}  // namespace aslhip

@@ -1,0 +1,874 @@
+// lattice.cpp — C ABI of the gfx950 LBGK library (include/aslhip.h).
+//
+// Host-side state of one lattice: device buffers, the boundary-condition
+// list, the driver's execute order, and the finalize step that turns ASL's
+// ghost-fill BCs into per-link resolution tables for the fused step kernel.
+//
+// How ASL's BC semantics map onto links.  ASL streams by pulling
+// f_k(x) <- f_k(x - c_k) after the BCs have overwritten some ghost entries
+// (g, k) (src/num/aslLBGK.cxx:237-250; src/num/aslLBGKBC.cxx).  So the value a
+// computation node x receives along direction k is the BC value of entry
+// (x - c_k, k) if some executed BC writes that entry (the last one executed
+// wins), and otherwise the collided population of x - c_k.  finalize()
+// resolves every incoming link of every computed cell that way, in the
+// driver's execute order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/aslhip.h"
+#include "kernels.hpp"
+#include "lattice_consts.hpp"
+
+using namespace aslhip;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) return fail(ASLHIP_EHIP, "ASL ERROR: %s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+struct Templ {
+  int Q;
+  int c[27][3];
+  double w[27];
+  int inv[27];
+};
+
+template <int Q>
+Templ make_templ_q() {
+  Templ t;
+  t.Q = Q;
+  for (int i = 0; i < Q; ++i) {
+    for (int d = 0; d < 3; ++d) t.c[i][d] = Lat<Q>::c[i][d];
+    t.w[i] = Lat<Q>::w(i);
+    t.inv[i] = inv_dir<Q>(i);
+  }
+  return t;
+}
+
+Templ make_templ(int q) {
+  if (q == 15) return make_templ_q<15>();
+  if (q == 27) return make_templ_q<27>();
+  return make_templ_q<19>();
+}
+
+struct BCDesc {
+  int kind = 0;
+  unsigned faces = 0;
+  double p = 0, u[3] = {0, 0, 0};
+  bool has_rot = false;
+  double rot[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<float> map, cd;
+  bool is_map() const { return kind >= ASLHIP_BC_NOSLIP_MAP; }
+};
+
+// v . c_i with zero terms left out, in type C.
+template <class C>
+C dot_c(const Templ &t, const C *u, int i) {
+  C a = C(0);
+  bool first = true;
+  for (int d = 0; d < 3; ++d) {
+    if (t.c[i][d] == 0) continue;
+    C v = t.c[i][d] > 0 ? u[d] : -u[d];
+    a = first ? v : a + v;
+    first = false;
+  }
+  return a;
+}
+
+}  // namespace
+
+struct aslhip_lattice {
+  int n[3] = {0, 0, 0};
+  int S[3] = {0, 0, 0};
+  int sx = 0, sy = 0, len = 0;
+  int Q = 19, dtype = ASLHIP_F32;
+  bool periodic = false;
+  Templ t;
+  size_t selem = 4, melem = 4, celem = 4;  // storage / macroscopic / compute element bytes
+  long long qs_bytes = 0;
+  hipStream_t stream = nullptr;
+  // device buffers
+  char *f = nullptr;
+  void *rho = nullptr, *u = nullptr;
+  unsigned char *code = nullptr;
+  unsigned short *pat = nullptr;
+  LinkRec *rec = nullptr;
+  unsigned char *mode = nullptr;
+  void *val = nullptr;
+  VLink *vl = nullptr;
+  VLinkBC *vlbc = nullptr;
+  double *red_partial = nullptr, *red_out = nullptr;
+  int nvl = 0;
+  // parameters
+  double nu = 0.1;
+  bool compressible = true, comp_rho = true, comp_vel = true;
+  double dx = 1., pos0[3] = {0., 0., 0.};
+  // BCs and order
+  std::vector<BCDesc> bcs;
+  std::vector<int> order, pending;
+  bool dirty = true;
+  bool initialized = false;
+  bool fresh = false;        // no step since init
+  bool bc_since_init = false;
+  bool next_odd = true;
+  // stats
+  long long n_counted = 0, n_computed = 0, n_boundary = 0, n_patterns = 0;
+  std::vector<unsigned char> h_code;
+
+  int idx(int x, int y, int z) const { return x * sx + y * sy + z; }
+};
+
+namespace {
+
+int check(aslhip_lattice *h) {
+  if (!h) return fail(ASLHIP_EINVAL, "ASL ERROR: null lattice handle");
+  return 0;
+}
+
+void free_all(aslhip_lattice *h) {
+  void *ptrs[] = {h->f, h->rho, h->u, h->code, h->pat, h->rec, h->mode, h->val, h->vl, h->vlbc, h->red_partial, h->red_out};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+// Relaxation rate w = 1/(3 nu + 1/2) in the compute type
+// (src/num/aslLBGK.cxx:203).
+double relaxation(const aslhip_lattice *h) {
+  if (h->dtype == ASLHIP_F64) return 1. / (3. * h->nu + .5);
+  const float nu = static_cast<float>(h->nu);
+  return static_cast<double>(1.f / (3.f * nu + .5f));
+}
+
+// Slot values of one BC in the compute type C.
+template <class C>
+void bc_values(const aslhip_lattice *h, const BCDesc &b, unsigned char &md, C *val) {
+  const Templ &t = h->t;
+  const C u[3] = {C(b.u[0]), C(b.u[1]), C(b.u[2])};
+  const C p = C(b.p);
+  for (int k = 0; k < t.Q; ++k) val[k] = C(0);
+  switch (b.kind) {
+    case ASLHIP_BC_NOSLIP:
+    case ASLHIP_BC_NOSLIP_MAP:
+      md = MODE_BOUNCE;
+      break;
+    case ASLHIP_BC_CONSTANT_VELOCITY:  // f_kbar(p) + 6.*(velocity*c_k)
+      md = MODE_BOUNCE;
+      for (int k = 1; k < t.Q; ++k) val[k] = C(6.) * dot_c(t, u, k);
+      break;
+    case ASLHIP_BC_CONSTANT_PRESSURE:
+      md = MODE_CONST;
+      for (int k = 1; k < t.Q; ++k) val[k] = p;
+      break;
+    case ASLHIP_BC_CONSTANT_PRESSURE_VELOCITY:  // pressure + 3.*(velocity*c_k)
+      md = MODE_CONST;
+      for (int k = 1; k < t.Q; ++k) val[k] = p + C(3.) * dot_c(t, u, k);
+      break;
+    case ASLHIP_BC_PRESSURE_VELOCITY_MAP: {  // src/num/aslLBGKBC.cxx:400-415
+      md = MODE_CONST;
+      C uu = u[0] * u[0];
+      uu = uu + u[1] * u[1];
+      uu = uu + u[2] * u[2];
+      for (int k = 1; k < t.Q; ++k) {
+        const C a = dot_c(t, u, k);
+        if (h->compressible) {
+          val[k] = ((p + (C(3.) * p) * a) + ((C(4.5) * p) * a) * a) - (C(1.5) * p) * uu;
+        } else {
+          const C u3[3] = {C(3.) * u[0], C(3.) * u[1], C(3.) * u[2]};
+          val[k] = ((p + dot_c(t, u3, k)) + (C(4.5) * a) * a) - C(1.5) * uu;
+        }
+      }
+      break;
+    }
+    case ASLHIP_BC_VELOCITY_MAP:
+      md = MODE_LINK;
+      break;
+    default:
+      md = MODE_NONE;
+  }
+}
+
+// Out-of-block-safe map read (src/acl/aclMath/aclMathAlg.cxx:84-94).
+inline float map_safe(const std::vector<float> &m, long long i, long long len, float out) {
+  return (i < 0 || i > len - 1) ? out : m[(size_t)i];
+}
+
+// isComputationNode (src/num/aslBCond.cxx:204-211): map > 0 [&& cdomain > 0].
+inline bool comp_node(const BCDesc &b, long long i, long long len) {
+  bool r = map_safe(b.map, i, len, -.9f) > 0.f;
+  if (!b.cd.empty()) r = r && map_safe(b.cd, i, len, -1.f) > 0.f;
+  return r;
+}
+
+struct PatHash {
+  size_t operator()(const std::string &s) const { return std::hash<std::string>()(s); }
+};
+
+int finalize_impl(aslhip_lattice *h) {
+  const Templ &t = h->t;
+  const int Q = t.Q, len = h->len;
+  std::vector<unsigned char> code(len, CODE_SKIP);
+  std::vector<unsigned short> pat(len, 0);
+  std::vector<LinkRec> recs(1);
+  std::memset(&recs[0], 0, sizeof(LinkRec));
+  std::vector<VLink> vlinks;
+  std::vector<VLinkBC> vlbc(256);
+  const int nord = static_cast<int>(h->order.size());
+  if (nord > 254) return fail(ASLHIP_EINVAL, "ASL ERROR: more than 254 boundary-condition executions per step");
+  int off[27];
+  for (int k = 0; k < Q; ++k) off[k] = t.c[k][0] * h->sx + t.c[k][1] * h->sy + t.c[k][2];
+  auto interior = [&](int x, int y, int z) {
+    return x >= 1 && x <= h->n[0] && y >= 1 && y <= h->n[1] && z >= 1 && z <= h->n[2];
+  };
+
+  h->n_counted = h->n_computed = h->n_boundary = 0;
+  if (h->periodic) {
+    if (nord > 0 || !h->bcs.empty())
+      return fail(ASLHIP_EINVAL, "ASL ERROR: boundary conditions on a periodic lattice");
+    for (int x = 1; x <= h->n[0]; ++x)
+      for (int y = 1; y <= h->n[1]; ++y)
+        for (int z = 1; z <= h->n[2]; ++z) code[h->idx(x, y, z)] = CODE_FLUID;
+    h->n_counted = h->n_computed = (long long)h->n[0] * h->n[1] * h->n[2];
+  } else {
+    // BC of each order position; map BCs applied.
+    std::vector<const BCDesc *> ob(nord);
+    for (int o = 0; o < nord; ++o) ob[o] = &h->bcs[h->order[o]];
+    std::vector<int> map_pos;
+    for (int o = 0; o < nord; ++o)
+      if (ob[o]->is_map()) map_pos.push_back(o);
+    static const int face_dir[6] = {1, 4, 2, 5, 3, 6};  // X0 XE Y0 YE Z0 ZE: vectors[1..6]
+
+    // Slot (order position + 1) of the last executed BC that writes entry
+    // (x - c_k, k), for interior x; 0 if none.
+    auto resolve = [&](int x, int y, int z, int k) -> int {
+      const int i = h->idx(x, y, z);
+      const int p[3] = {x, y, z};
+      for (int o = nord - 1; o >= 0; --o) {
+        const BCDesc &b = *ob[o];
+        if (!b.is_map()) {
+          // addSliceX0 ... addSliceZE (src/num/aslBCond.cxx:373-493)
+          for (int fc = 0; fc < 6; ++fc) {
+            if (!(b.faces & (1u << fc))) continue;
+            const int axis = fc / 2;
+            const int layer = (fc % 2 == 0) ? 1 : h->S[axis] - 2;
+            const int *e = t.c[face_dir[fc]];
+            if (p[axis] == layer && t.c[k][0] * e[0] + t.c[k][1] * e[1] + t.c[k][2] * e[2] > 0) return o + 1;
+          }
+        } else {
+          // map guard and predicates (src/num/aslLBGKBC.cxx:235-251)
+          const float mg = b.map[(size_t)(i - off[k])];
+          if (mg <= 0.f && mg > -.9999f && comp_node(b, i, len)) return o + 1;
+        }
+      }
+      return 0;
+    };
+
+    std::vector<unsigned char> computed(len, 0), counted(len, 0);
+    for (int x = 1; x <= h->n[0]; ++x)
+      for (int y = 1; y <= h->n[1]; ++y)
+        for (int z = 1; z <= h->n[2]; ++z) {
+          const int i = h->idx(x, y, z);
+          bool solid = false;
+          for (int o : map_pos)
+            if (ob[o]->map[(size_t)i] <= 0.f) solid = true;
+          computed[i] = counted[i] = solid ? 0 : 1;
+        }
+    // Closure: a cell that is not computed but sends an unresolved link into
+    // a computed cell must be computed too (ASL collides every cell).
+    for (bool changed = true; changed;) {
+      changed = false;
+      for (int x = 0; x < h->S[0]; ++x)
+        for (int y = 0; y < h->S[1]; ++y)
+          for (int z = 0; z < h->S[2]; ++z) {
+            const int s = h->idx(x, y, z);
+            if (computed[s]) continue;
+            for (int k = 1; k < Q; ++k) {
+              const int X = x + t.c[k][0], Y = y + t.c[k][1], Z = z + t.c[k][2];
+              if (!interior(X, Y, Z)) continue;
+              if (!computed[h->idx(X, Y, Z)] || resolve(X, Y, Z, k) != 0) continue;
+              if (!interior(x, y, z))
+                return fail(ASLHIP_EUNDEF,
+                            "ASL ERROR: no boundary condition fills the ghost link (%d,%d,%d) -> (%d,%d,%d) "
+                            "(ASL would stream uninitialised memory into the block)",
+                            x - 1, y - 1, z - 1, X - 1, Y - 1, Z - 1);
+              computed[s] = 1;
+              changed = true;
+              break;
+            }
+          }
+    }
+
+    // Cells that some map BC treats as ghost nodes (map <= 0), any layer.
+    std::vector<unsigned char> solid_any(len, 0);
+    for (int o : map_pos)
+      for (int i = 0; i < len; ++i)
+        if (ob[o]->map[(size_t)i] <= 0.f) solid_any[i] = 1;
+    std::unordered_map<std::string, int, PatHash> pid;
+    for (int x = 1; x <= h->n[0]; ++x)
+      for (int y = 1; y <= h->n[1]; ++y)
+        for (int z = 1; z <= h->n[2]; ++z) {
+          const int i = h->idx(x, y, z);
+          if (!computed[i]) continue;
+          LinkRec r;
+          std::memset(&r, 0, sizeof r);
+          bool any = false;
+          // A link into x can only be BC-resolved if x is in a face layer or
+          // its source x - c_k is a map ghost node.
+          bool near = x == 1 || x == h->n[0] || y == 1 || y == h->n[1] || z == 1 || z == h->n[2];
+          for (int k = 1; k < Q && !near && !map_pos.empty(); ++k) near = solid_any[i - off[k]] != 0;
+          for (int k = 1; k < Q && near; ++k) {
+            const int s = resolve(x, y, z, k);
+            r.slot[k] = static_cast<unsigned char>(s);
+            any = any || s != 0;
+          }
+          // Outgoing link k of a computed ghost cell: skip it if its receiver
+          // resolves it (computation nodes are never the source of a BC entry).
+          for (int k = 1; k < Q && !counted[i]; ++k) {
+            const int X = x + t.c[k][0], Y = y + t.c[k][1], Z = z + t.c[k][2];
+            if (interior(X, Y, Z) && computed[h->idx(X, Y, Z)] && resolve(X, Y, Z, k) != 0) {
+              r.skip |= 1u << k;
+              any = true;
+            }
+          }
+          ++h->n_computed;
+          if (counted[i]) ++h->n_counted;
+          if (!any && counted[i]) {
+            code[i] = CODE_FLUID;
+            continue;
+          }
+          code[i] = counted[i] ? CODE_FLUID_BC : CODE_GHOST;
+          if (counted[i]) ++h->n_boundary;
+          std::string key(reinterpret_cast<const char *>(&r), sizeof r);
+          auto it = pid.find(key);
+          int id;
+          if (it == pid.end()) {
+            id = static_cast<int>(recs.size());
+            if (id > 65535) return fail(ASLHIP_EINVAL, "ASL ERROR: more than 65535 distinct boundary link patterns");
+            recs.push_back(r);
+            pid.emplace(key, id);
+          } else {
+            id = it->second;
+          }
+          pat[i] = static_cast<unsigned short>(id);
+        }
+
+    // Boundary-link entries of velocity map BCs.
+    for (int o = 0; o < nord; ++o) {
+      const BCDesc &b = *ob[o];
+      if (b.kind != ASLHIP_BC_VELOCITY_MAP) continue;
+      VLinkBC &vb = vlbc[o + 1];
+      for (int d = 0; d < 3; ++d) {
+        vb.u[d] = b.u[d];
+        vb.omega[d] = b.rot[d];
+        vb.centre[d] = b.rot[3 + d];
+      }
+      vb.has_rot = b.has_rot ? 1 : 0;
+      for (int g = 0; g < len; ++g) {
+        const float mg = b.map[(size_t)g];
+        if (!(mg <= 0.f && mg > -.9999f)) continue;
+        VLink l{g, 0u, 0u, o + 1};
+        const int gx = g / h->sx, gy = (g % h->sx) / h->sy, gz = g % h->sy;
+        for (int i = 1; i < Q; ++i) {
+          const long long p = (long long)g + off[i];
+          if (!comp_node(b, p, len)) continue;
+          l.comp |= 1u << i;
+          const int X = gx + t.c[i][0], Y = gy + t.c[i][1], Z = gz + t.c[i][2];
+          if (interior(X, Y, Z) && computed[h->idx(X, Y, Z)] && h->idx(X, Y, Z) == p &&
+              resolve(X, Y, Z, i) == o + 1)
+            l.own |= 1u << i;
+        }
+        if (!l.own) continue;
+        for (int i = 1; i < Q; ++i) {
+          if (!((l.comp >> i) & 1u)) continue;
+          const int X = gx + t.c[i][0], Y = gy + t.c[i][1], Z = gz + t.c[i][2];
+          if (!interior(X, Y, Z) || !computed[h->idx(X, Y, Z)])
+            return fail(ASLHIP_EUNDEF,
+                        "ASL ERROR: velocity map BC at ghost (%d,%d,%d) averages a neighbour that is not a "
+                        "computed cell",
+                        gx - 1, gy - 1, gz - 1);
+        }
+        vlinks.push_back(l);
+      }
+    }
+  }
+  h->n_patterns = static_cast<long long>(recs.size()) - 1;
+
+  // Slot tables.
+  std::vector<unsigned char> mode(256, MODE_NONE);
+  std::vector<char> val(256 * (size_t)Q * h->celem, 0);
+  for (int o = 0; o < nord; ++o) {
+    const BCDesc &b = h->bcs[h->order[o]];
+    if (h->celem == 8)
+      bc_values<double>(h, b, mode[o + 1], reinterpret_cast<double *>(val.data()) + (size_t)(o + 1) * Q);
+    else
+      bc_values<float>(h, b, mode[o + 1], reinterpret_cast<float *>(val.data()) + (size_t)(o + 1) * Q);
+  }
+
+  // Upload.
+  HIPCHK(hipMemcpyAsync(h->code, code.data(), len, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->pat, pat.data(), (size_t)len * 2, hipMemcpyHostToDevice, h->stream));
+  if (h->rec) HIPCHK(hipFree(h->rec));
+  h->rec = nullptr;
+  HIPCHK(hipMalloc(&h->rec, recs.size() * sizeof(LinkRec)));
+  HIPCHK(hipMemcpyAsync(h->rec, recs.data(), recs.size() * sizeof(LinkRec), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->mode, mode.data(), 256, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->val, val.data(), val.size(), hipMemcpyHostToDevice, h->stream));
+  if (h->vl) HIPCHK(hipFree(h->vl));
+  h->vl = nullptr;
+  h->nvl = static_cast<int>(vlinks.size());
+  if (h->nvl) {
+    HIPCHK(hipMalloc(&h->vl, vlinks.size() * sizeof(VLink)));
+    HIPCHK(hipMemcpyAsync(h->vl, vlinks.data(), vlinks.size() * sizeof(VLink), hipMemcpyHostToDevice, h->stream));
+  }
+  HIPCHK(hipMemcpyAsync(h->vlbc, vlbc.data(), 256 * sizeof(VLinkBC), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->h_code.swap(code);
+  h->dirty = false;
+  return 0;
+}
+
+int run_vlinks(aslhip_lattice *h, bool after_odd) {
+  if (!h->nvl) return 0;
+  VLinkArgs a;
+  a.f = h->f;
+  a.qs_bytes = h->qs_bytes;
+  a.sx = h->sx;
+  a.sy = h->sy;
+  a.links = h->vl;
+  a.nlinks = h->nvl;
+  a.bc = h->vlbc;
+  a.dx = h->dx;
+  for (int d = 0; d < 3; ++d) a.pos0[d] = h->pos0[d];
+  a.after_odd = after_odd ? 1 : 0;
+  HIPCHK(launch_vlinks(h->Q, h->dtype, a, h->stream));
+  return 0;
+}
+
+int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEvent_t> *ev) {
+  if (nsteps < 0) return fail(ASLHIP_EINVAL, "ASL ERROR: negative step count");
+  if (!h->initialized) return fail(ASLHIP_ESTATE, "ASL ERROR: LBGK::execute before initF");
+  if (!h->pending.empty()) {
+    if (h->pending != h->order) {
+      h->order = h->pending;
+      h->dirty = true;
+    }
+    h->pending.clear();
+  }
+  if (h->dirty) {
+    int r = finalize_impl(h);
+    if (r) return r;
+  }
+  if (nsteps == 0) return 0;
+  bool raw = false;
+  if (h->fresh) {
+    raw = !h->bc_since_init;
+    if (!raw) {
+      int r = run_vlinks(h, false);
+      if (r) return r;
+    }
+  }
+  StepArgs a;
+  a.f = h->f;
+  a.qs_bytes = h->qs_bytes;
+  a.sx = h->sx;
+  a.sy = h->sy;
+  a.nx = h->n[0];
+  a.ny = h->n[1];
+  a.nz = h->n[2];
+  a.len = h->len;
+  a.bpp = (h->n[1] * h->sy + 255) / 256;
+  a.nblocks = a.bpp * h->n[0];
+  a.code = h->code;
+  a.pat = h->pat;
+  a.rec = h->rec;
+  a.mode = h->mode;
+  a.val = h->val;
+  a.rho = h->rho;
+  a.u = h->u;
+  a.w = relaxation(h);
+  for (int s = 0; s < nsteps; ++s) {
+    unsigned fl = h->compressible ? STEP_F_COMPRESSIBLE : 0u;
+    if (s == 0 && raw) fl |= STEP_F_RAW;
+    if (s == nsteps - 1 && (flags & ASLHIP_STEP_WRITE_MACRO)) {
+      if (h->comp_rho) fl |= STEP_F_RHO;
+      if (h->comp_vel) fl |= STEP_F_VEL;
+    }
+    a.flags = fl;
+    if (ev) HIPCHK(hipEventRecord((*ev)[2 * s], h->stream));
+    HIPCHK(launch_step(h->Q, h->dtype, h->next_odd, h->periodic, a, h->stream));
+    if (ev) HIPCHK(hipEventRecord((*ev)[2 * s + 1], h->stream));
+    const bool was_odd = h->next_odd;
+    h->next_odd = !h->next_odd;
+    int r = run_vlinks(h, was_odd);
+    if (r) return r;
+  }
+  h->fresh = false;
+  return 0;
+}
+
+int copy_out(aslhip_lattice *h, const void *dev, void *host, size_t bytes) {
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aslhip_last_error(char *buf, size_t n) {
+  if (buf && n) {
+    std::strncpy(buf, g_err.c_str(), n - 1);
+    buf[n - 1] = 0;
+  }
+  return static_cast<int>(g_err.size());
+}
+
+int aslhip_device_info(char *buf, size_t n, int *cu_count, double *hbm_bytes) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, dev));
+  if (buf && n) {
+    std::snprintf(buf, n, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (cu_count) *cu_count = p.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = static_cast<double>(p.totalGlobalMem);
+  return 0;
+}
+
+int aslhip_lattice_create(aslhip_lattice **out, int nx, int ny, int nz, int q, int dtype, unsigned flags) {
+  if (!out) return fail(ASLHIP_EINVAL, "ASL ERROR: null output handle");
+  *out = nullptr;
+  if (nx < 1 || ny < 1 || nz < 1) return fail(ASLHIP_EINVAL, "ASL ERROR: block size must be positive");
+  if (q != 15 && q != 19 && q != 27) return fail(ASLHIP_EINVAL, "ASL ERROR: lattice template must be D3Q15, D3Q19 or D3Q27");
+  if (dtype < ASLHIP_F32 || dtype > ASLHIP_F16) return fail(ASLHIP_EINVAL, "ASL ERROR: unknown population type %d", dtype);
+  const unsigned per = flags & 7u;
+  if (per != 0 && per != 7) return fail(ASLHIP_EINVAL, "ASL ERROR: periodic wrap must be on all three axes");
+  const long long S0 = nx + 2LL, S1 = ny + 2LL, S2 = nz + 2LL;
+  const long long len = S0 * S1 * S2;
+  if (len > 0x7fffffffLL - 4096) return fail(ASLHIP_EINVAL, "ASL ERROR: block too large for one lattice");
+  aslhip_lattice *h = new aslhip_lattice();
+  h->n[0] = nx;
+  h->n[1] = ny;
+  h->n[2] = nz;
+  h->S[0] = (int)S0;
+  h->S[1] = (int)S1;
+  h->S[2] = (int)S2;
+  h->sy = (int)S2;
+  h->sx = (int)(S1 * S2);
+  h->len = (int)len;
+  h->Q = q;
+  h->dtype = dtype;
+  h->periodic = per == 7;
+  h->t = make_templ(q);
+  h->selem = dtype == ASLHIP_F64 ? 8 : (dtype == ASLHIP_F32 ? 4 : 2);
+  h->melem = h->celem = dtype == ASLHIP_F64 ? 8 : 4;
+  // direction arrays 256-byte aligned
+  const long long qs_elems = (len + 127) / 128 * 128;
+  h->qs_bytes = qs_elems * (long long)h->selem;
+  hipError_t e = hipStreamCreate(&h->stream);
+  if (e == hipSuccess) e = hipMalloc(&h->f, (size_t)h->qs_bytes * q);
+  if (e == hipSuccess) e = hipMalloc(&h->rho, (size_t)len * h->melem);
+  if (e == hipSuccess) e = hipMalloc(&h->u, (size_t)len * 3 * h->melem);
+  if (e == hipSuccess) e = hipMalloc(&h->code, (size_t)len);
+  if (e == hipSuccess) e = hipMalloc(&h->pat, (size_t)len * 2);
+  if (e == hipSuccess) e = hipMalloc(&h->mode, 256);
+  if (e == hipSuccess) e = hipMalloc(&h->val, 256 * (size_t)q * h->celem);
+  if (e == hipSuccess) e = hipMalloc(&h->vlbc, 256 * sizeof(VLinkBC));
+  if (e == hipSuccess) e = hipMalloc(&h->red_partial, (size_t)reduce_blocks((int)len) * 5 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&h->red_out, 5 * sizeof(double));
+  if (e != hipSuccess) {
+    free_all(h);
+    delete h;
+    return fail(e == hipErrorOutOfMemory ? ASLHIP_ENOMEM : ASLHIP_EHIP, "ASL ERROR: lattice allocation: %s",
+                hipGetErrorString(e));
+  }
+  *out = h;
+  return 0;
+}
+
+int aslhip_lattice_destroy(aslhip_lattice *h) {
+  if (!h) return 0;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_all(h);
+  delete h;
+  return 0;
+}
+
+int aslhip_set_viscosity(aslhip_lattice *h, double nu) {
+  if (int r = check(h)) return r;
+  if (!(nu >= 0.) && !std::isinf(nu)) return fail(ASLHIP_EINVAL, "ASL ERROR: viscosity must be >= 0");
+  h->nu = nu;
+  return 0;
+}
+
+int aslhip_set_compressible(aslhip_lattice *h, int c) {
+  if (int r = check(h)) return r;
+  h->compressible = c != 0;
+  h->dirty = true;  // pressure-velocity map values depend on it
+  return 0;
+}
+
+int aslhip_set_compute_macro(aslhip_lattice *h, int comp_rho, int comp_vel) {
+  if (int r = check(h)) return r;
+  h->comp_rho = comp_rho != 0;
+  h->comp_vel = comp_vel != 0;
+  return 0;
+}
+
+int aslhip_init_equilibrium(aslhip_lattice *h, double rho, const double u[3]) {
+  if (int r = check(h)) return r;
+  InitArgs a;
+  a.f = h->f;
+  a.qs_bytes = h->qs_bytes;
+  a.len = h->len;
+  a.rho0 = rho;
+  for (int d = 0; d < 3; ++d) a.u0[d] = u ? u[d] : 0.;
+  a.rho_in = a.u_in = nullptr;
+  a.rho = h->rho;
+  a.u = h->u;
+  HIPCHK(launch_init(h->Q, h->dtype, a, h->stream));
+  h->initialized = true;
+  h->fresh = true;
+  h->bc_since_init = false;
+  h->next_odd = true;
+  h->pending.clear();
+  return 0;
+}
+
+int aslhip_init_equilibrium_field(aslhip_lattice *h, const void *rho, const void *u) {
+  if (int r = check(h)) return r;
+  if (!u) return fail(ASLHIP_EINVAL, "ASL ERROR: initF needs a velocity field");
+  void *drho = nullptr, *du = nullptr;
+  const size_t n = (size_t)h->len * h->melem;
+  HIPCHK(hipMalloc(&du, 3 * n));
+  HIPCHK(hipMemcpyAsync(du, u, 3 * n, hipMemcpyHostToDevice, h->stream));
+  if (rho) {
+    HIPCHK(hipMalloc(&drho, n));
+    HIPCHK(hipMemcpyAsync(drho, rho, n, hipMemcpyHostToDevice, h->stream));
+  }
+  InitArgs a;
+  a.f = h->f;
+  a.qs_bytes = h->qs_bytes;
+  a.len = h->len;
+  a.rho0 = 1.;
+  a.u0[0] = a.u0[1] = a.u0[2] = 0.;
+  a.rho_in = drho;
+  a.u_in = du;
+  a.rho = h->rho;
+  a.u = h->u;
+  HIPCHK(launch_init(h->Q, h->dtype, a, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipFree(du));
+  if (drho) HIPCHK(hipFree(drho));
+  h->initialized = true;
+  h->fresh = true;
+  h->bc_since_init = false;
+  h->next_odd = true;
+  h->pending.clear();
+  return 0;
+}
+
+int aslhip_bc_add_slice(aslhip_lattice *h, int kind, unsigned faces, double p, const double u[3], int *bc_id) {
+  if (int r = check(h)) return r;
+  if (h->periodic) return fail(ASLHIP_EINVAL, "ASL ERROR: slice BCs need a non-periodic lattice");
+  if (kind < ASLHIP_BC_NOSLIP || kind > ASLHIP_BC_CONSTANT_PRESSURE_VELOCITY)
+    return fail(ASLHIP_EINVAL, "ASL ERROR: %d is not a slice BC kind", kind);
+  if (faces == 0 || faces > 63u) return fail(ASLHIP_EINVAL, "ASL ERROR: bad face mask %u", faces);
+  BCDesc b;
+  b.kind = kind;
+  b.faces = faces;
+  b.p = p;
+  for (int d = 0; d < 3; ++d) b.u[d] = u ? u[d] : 0.;
+  h->bcs.push_back(std::move(b));
+  if (bc_id) *bc_id = static_cast<int>(h->bcs.size()) - 1;
+  h->dirty = true;
+  return 0;
+}
+
+int aslhip_bc_add_map(aslhip_lattice *h, int kind, const float *map, const float *cdomain, double p,
+                      const double u[3], const double rot[6], int *bc_id) {
+  if (int r = check(h)) return r;
+  if (h->periodic) return fail(ASLHIP_EINVAL, "ASL ERROR: map BCs need a non-periodic lattice");
+  if (kind < ASLHIP_BC_NOSLIP_MAP || kind > ASLHIP_BC_VELOCITY_MAP)
+    return fail(ASLHIP_EINVAL, "ASL ERROR: %d is not a map BC kind", kind);
+  if (!map) return fail(ASLHIP_EINVAL, "ASL ERROR: map BC without a map");
+  BCDesc b;
+  b.kind = kind;
+  b.p = p;
+  for (int d = 0; d < 3; ++d) b.u[d] = u ? u[d] : 0.;
+  if (rot) {
+    b.has_rot = true;
+    for (int d = 0; d < 6; ++d) b.rot[d] = rot[d];
+  }
+  b.map.assign(map, map + h->len);
+  if (cdomain) b.cd.assign(cdomain, cdomain + h->len);
+  h->bcs.push_back(std::move(b));
+  if (bc_id) *bc_id = static_cast<int>(h->bcs.size()) - 1;
+  h->dirty = true;
+  return 0;
+}
+
+int aslhip_bc_execute(aslhip_lattice *h, int bc_id) {
+  if (int r = check(h)) return r;
+  if (bc_id < 0 || bc_id >= (int)h->bcs.size()) return fail(ASLHIP_EINVAL, "ASL ERROR: unknown BC id %d", bc_id);
+  // a BC executed twice in one cycle counts at its last position
+  auto it = std::find(h->pending.begin(), h->pending.end(), bc_id);
+  if (it != h->pending.end()) h->pending.erase(it);
+  h->pending.push_back(bc_id);
+  if (h->fresh) h->bc_since_init = true;
+  return 0;
+}
+
+int aslhip_finalize(aslhip_lattice *h) {
+  if (int r = check(h)) return r;
+  if (!h->pending.empty()) {
+    if (h->pending != h->order) {
+      h->order = h->pending;
+      h->dirty = true;
+    }
+    h->pending.clear();
+  }
+  return h->dirty ? finalize_impl(h) : 0;
+}
+
+int aslhip_step(aslhip_lattice *h, int nsteps, unsigned flags) {
+  if (int r = check(h)) return r;
+  return step_impl(h, nsteps, flags, nullptr);
+}
+
+int aslhip_synchronize(aslhip_lattice *h) {
+  if (int r = check(h)) return r;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int aslhip_read_rho(aslhip_lattice *h, void *out) {
+  if (int r = check(h)) return r;
+  return copy_out(h, h->rho, out, (size_t)h->len * h->melem);
+}
+
+int aslhip_read_velocity(aslhip_lattice *h, void *out) {
+  if (int r = check(h)) return r;
+  return copy_out(h, h->u, out, (size_t)h->len * 3 * h->melem);
+}
+
+int aslhip_read_f(aslhip_lattice *h, void *out) {
+  if (int r = check(h)) return r;
+  void *d = nullptr;
+  const size_t bytes = (size_t)h->len * h->Q * h->melem;
+  HIPCHK(hipMalloc(&d, bytes));
+  UnpermuteArgs a;
+  a.f = h->f;
+  a.qs_bytes = h->qs_bytes;
+  a.sx = h->sx;
+  a.sy = h->sy;
+  a.len = h->len;
+  a.after_odd = h->next_odd ? 0 : 1;
+  a.out = d;
+  HIPCHK(launch_unpermute(h->Q, h->dtype, a, h->stream));
+  int r = copy_out(h, d, out, bytes);
+  HIPCHK(hipFree(d));
+  return r;
+}
+
+int aslhip_read_cell_types(aslhip_lattice *h, uint8_t *out) {
+  if (int r = check(h)) return r;
+  if (h->dirty) {
+    int r = aslhip_finalize(h);
+    if (r) return r;
+  }
+  return copy_out(h, h->code, out, (size_t)h->len);
+}
+
+int aslhip_reduce_conservation(aslhip_lattice *h, double out[5]) {
+  if (int r = check(h)) return r;
+  if (h->dirty) {
+    int r = aslhip_finalize(h);
+    if (r) return r;
+  }
+  ReduceArgs a;
+  a.f = h->f;
+  a.qs_bytes = h->qs_bytes;
+  a.sx = h->sx;
+  a.sy = h->sy;
+  a.len = h->len;
+  a.after_odd = h->next_odd ? 0 : 1;
+  a.code = h->code;
+  a.partial = h->red_partial;
+  HIPCHK(launch_reduce(h->Q, h->dtype, a, h->red_out, h->stream));
+  return copy_out(h, h->red_out, out, 5 * sizeof(double));
+}
+
+int aslhip_stats(aslhip_lattice *h, int64_t out[4]) {
+  if (int r = check(h)) return r;
+  if (h->dirty) {
+    int r = aslhip_finalize(h);
+    if (r) return r;
+  }
+  out[0] = h->n_counted;
+  out[1] = h->n_computed;
+  out[2] = h->n_boundary;
+  out[3] = h->n_patterns;
+  return 0;
+}
+
+int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags, double *ms_total, double *ms_kernel) {
+  if (int r = check(h)) return r;
+  if (nsteps < 1) return fail(ASLHIP_EINVAL, "ASL ERROR: time_steps needs at least one step");
+  if (h->dirty || !h->pending.empty()) {
+    int r = aslhip_finalize(h);
+    if (r) return r;
+  }
+  std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
+  for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+  hipEvent_t t0, t1;
+  HIPCHK(hipEventCreate(&t0));
+  HIPCHK(hipEventCreate(&t1));
+  HIPCHK(hipEventRecord(t0, h->stream));
+  int r = step_impl(h, nsteps, flags, &ev);
+  HIPCHK(hipEventRecord(t1, h->stream));
+  HIPCHK(hipEventSynchronize(t1));
+  if (r == 0) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+    if (ms_total) *ms_total = ms;
+    double ksum = 0.;
+    for (int s = 0; s < nsteps; ++s) {
+      float k = 0.f;
+      HIPCHK(hipEventElapsedTime(&k, ev[2 * s], ev[2 * s + 1]));
+      ksum += k;
+    }
+    if (ms_kernel) *ms_kernel = ksum;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,309 @@
+"""Host-side mirror of ASL's LBGK interface over the C ABI.
+
+The names follow ASL's driver-facing API (``src/num/aslLBGK.h``,
+``src/num/aslLBGKBC.h``, ``src/num/aslNumMethod.h``): ``LBGK`` with
+``init()`` / ``execute()`` / ``getRho()`` / ``getVelocity()`` / ``getF()``,
+``LBGKUtilities.initF``, and the BC factories ``generateBCNoSlip``,
+``generateBCConstantPressure``, ``generateBCConstantVelocity``,
+``generateBCConstantPressureVelocity``, ``generateBCVelocity`` with
+``initAll`` / ``executeAll``.  Host views use ASL's ghost-inclusive,
+z-fastest layout, as numpy arrays of shape (nx+2, ny+2, nz+2).
+
+``Lattice`` is the thin handle wrapper underneath; ``LBGK.execute()`` keeps
+one step pending so that the step that precedes a host read is the only one
+that writes rho and u (what ASL's per-step writes leave visible is exactly
+that step's output).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (ASLError, BC_CONSTANT_PRESSURE, BC_CONSTANT_PRESSURE_VELOCITY, BC_CONSTANT_VELOCITY,
+                   BC_NOSLIP, BC_NOSLIP_MAP, BC_PRESSURE_VELOCITY_MAP, BC_VELOCITY_MAP, BF16, F16, F32, F64,
+                   STEP_WRITE_MACRO, X0, XE, Y0, YE, Z0, ZE, call)
+
+_DT = {"f32": F32, "f64": F64, "bf16": BF16, "f16": F16, F32: F32, F64: F64, BF16: BF16, F16: F16}
+
+
+def _d3(u):
+    a = np.ascontiguousarray(np.asarray((0.0, 0.0, 0.0) if u is None else u, dtype=np.float64).reshape(3))
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Lattice:
+    """One lattice handle (``aslhip_lattice``): internal block ``size``, template ``q``."""
+
+    def __init__(self, size, q=19, dtype="f32", periodic=False, nu=None):
+        _lib.load_library()
+        self.size = tuple(int(s) for s in size)
+        self.S = tuple(s + 2 for s in self.size)
+        self.len = int(np.prod(self.S))
+        self.q = int(q)
+        self.dtype = _DT[dtype]
+        self.host_dtype = np.float64 if self.dtype == F64 else np.float32
+        h = ctypes.c_void_p()
+        call("aslhip_lattice_create", ctypes.byref(h), *self.size, self.q, self.dtype, 7 if periodic else 0)
+        self.h = h
+        self._keep = []
+        if nu is not None:
+            self.set_viscosity(nu)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load_library().aslhip_lattice_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # parameters ---------------------------------------------------------
+    def set_viscosity(self, nu):
+        call("aslhip_set_viscosity", self.h, float(nu))
+
+    def set_compressible(self, flag=True):
+        call("aslhip_set_compressible", self.h, 1 if flag else 0)
+
+    def set_compute_macro(self, rho=True, vel=True):
+        call("aslhip_set_compute_macro", self.h, int(bool(rho)), int(bool(vel)))
+
+    # initialisation -----------------------------------------------------
+    def init_equilibrium(self, rho=1.0, u=(0.0, 0.0, 0.0)):
+        if np.ndim(rho) == 0 and np.ndim(u) == 1:
+            _, up = _d3(u)
+            call("aslhip_init_equilibrium", self.h, float(rho), up)
+            return
+        uf = np.ascontiguousarray(np.asarray(u, dtype=self.host_dtype).reshape(3 * self.len))
+        rf = None
+        if np.ndim(rho) != 0 or float(rho) != 1.0:
+            rf = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, dtype=self.host_dtype), self.S).reshape(self.len))
+        call("aslhip_init_equilibrium_field", self.h, None if rf is None else rf.ctypes.data, uf.ctypes.data)
+
+    # boundary conditions --------------------------------------------------
+    def add_slice_bc(self, kind, faces, p=0.0, u=None):
+        _, up = _d3(u)
+        bid = ctypes.c_int(-1)
+        call("aslhip_bc_add_slice", self.h, int(kind), int(faces), float(p), up, ctypes.byref(bid))
+        return bid.value
+
+    def add_map_bc(self, kind, mapv, cdomain=None, p=0.0, u=None, rot=None):
+        m = np.ascontiguousarray(np.asarray(mapv, dtype=np.float32).reshape(self.len))
+        cd = None if cdomain is None else np.ascontiguousarray(np.asarray(cdomain, dtype=np.float32).reshape(self.len))
+        _, up = _d3(u)
+        rr = None if rot is None else np.ascontiguousarray(np.asarray(rot, dtype=np.float64).reshape(6))
+        bid = ctypes.c_int(-1)
+        fp = ctypes.POINTER(ctypes.c_float)
+        call("aslhip_bc_add_map", self.h, int(kind), m.ctypes.data_as(fp), None if cd is None else cd.ctypes.data_as(fp),
+             float(p), up, None if rr is None else rr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+             ctypes.byref(bid))
+        return bid.value
+
+    def execute_bc(self, bc_id):
+        call("aslhip_bc_execute", self.h, int(bc_id))
+
+    def finalize(self):
+        call("aslhip_finalize", self.h)
+
+    # stepping -------------------------------------------------------------
+    def step(self, nsteps=1, write_macro=True):
+        call("aslhip_step", self.h, int(nsteps), STEP_WRITE_MACRO if write_macro else 0)
+
+    def synchronize(self):
+        call("aslhip_synchronize", self.h)
+
+    def time_steps(self, nsteps, write_macro=False):
+        tot = ctypes.c_double(0)
+        ker = ctypes.c_double(0)
+        call("aslhip_time_steps", self.h, int(nsteps), STEP_WRITE_MACRO if write_macro else 0, ctypes.byref(tot),
+             ctypes.byref(ker))
+        return tot.value, ker.value
+
+    # host views -----------------------------------------------------------
+    def rho(self):
+        out = np.empty(self.len, dtype=self.host_dtype)
+        call("aslhip_read_rho", self.h, out.ctypes.data)
+        return out.reshape(self.S)
+
+    def velocity(self):
+        out = np.empty(3 * self.len, dtype=self.host_dtype)
+        call("aslhip_read_velocity", self.h, out.ctypes.data)
+        return out.reshape((3,) + self.S)
+
+    def f(self):
+        out = np.empty(self.q * self.len, dtype=self.host_dtype)
+        call("aslhip_read_f", self.h, out.ctypes.data)
+        return out.reshape((self.q,) + self.S)
+
+    def cell_types(self):
+        out = np.empty(self.len, dtype=np.uint8)
+        call("aslhip_read_cell_types", self.h, out.ctypes.data)
+        return out.reshape(self.S)
+
+    def conservation(self):
+        out = np.empty(5)
+        call("aslhip_reduce_conservation", self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    def stats(self):
+        out = (ctypes.c_int64 * 4)()
+        call("aslhip_stats", self.h, out)
+        return {"counted": out[0], "computed": out[1], "boundary": out[2], "patterns": out[3]}
+
+
+# ---------------------------------------------------------------------------
+# ASL-named layer (src/num/aslLBGK.h, src/num/aslLBGKBC.h)
+# ---------------------------------------------------------------------------
+class VectorTemplate:
+    """``asl::VectorTemplate`` handle for d3q15() / d3q19() / d3q27()."""
+
+    def __init__(self, q):
+        self.q = q
+
+
+def d3q15():
+    return VectorTemplate(15)
+
+
+def d3q19():
+    return VectorTemplate(19)
+
+
+def d3q27():
+    return VectorTemplate(27)
+
+
+class Block:
+    """``asl::Block`` (src/data/aslBlocks.h:56-88): size, dx, position."""
+
+    def __init__(self, size, dx=1.0, position=(0.0, 0.0, 0.0)):
+        self.size = tuple(int(s) for s in size)
+        self.dx = float(dx)
+        self.position = tuple(float(p) for p in position)
+
+
+class NumMethod:
+    """``asl::NumMethod`` (src/num/aslNumMethod.h:34-42)."""
+
+    def init(self):
+        raise NotImplementedError
+
+    def execute(self):
+        raise NotImplementedError
+
+
+def initAll(v):
+    for m in v:
+        m.init()
+
+
+def executeAll(v):
+    for m in v:
+        m.execute()
+
+
+class LBGK(NumMethod):
+    """``asl::LBGK(Block, nu, const VectorTemplate*, compVel, compRho)``."""
+
+    def __init__(self, block, nu, vt=None, compVel=True, compRho=True, dtype="f32", periodic=False):
+        vt = vt or d3q19()
+        self.block = block
+        self.vectorTemplate = vt
+        self.lattice = Lattice(block.size, vt.q, dtype=dtype, periodic=periodic, nu=nu)
+        self.lattice.set_compute_macro(compRho, compVel)
+        self._pending = 0
+
+    def init(self):
+        pass
+
+    def setCompressible(self, flag=True):
+        self.lattice.set_compressible(flag)
+
+    def execute(self):
+        self._flush(write_macro=False)
+        self._pending = 1
+
+    def _flush(self, write_macro=True):
+        if self._pending:
+            self.lattice.step(self._pending, write_macro=write_macro)
+            self._pending = 0
+
+    def getRho(self):
+        self._flush()
+        return self.lattice.rho()
+
+    def getVelocity(self):
+        self._flush()
+        return self.lattice.velocity()
+
+    def getF(self):
+        self._flush()
+        return self.lattice.f()
+
+    def getVectorTemplate(self):
+        return self.vectorTemplate
+
+
+class LBGKUtilities:
+    """``asl::LBGKUtilities(SPLBGK)::initF`` (src/num/aslLBGK.cxx:262-280)."""
+
+    def __init__(self, lbgk):
+        self.num = lbgk
+
+    def initF(self, *args):
+        if len(args) == 1:
+            rho, vel = 1.0, args[0]
+        else:
+            rho, vel = args
+        self.num._pending = 0
+        self.num.lattice.init_equilibrium(rho, vel)
+
+
+class _BC(NumMethod):
+    def __init__(self, lbgk, bc_id):
+        self.num = lbgk
+        self.id = bc_id
+
+    def init(self):
+        pass
+
+    def execute(self):
+        self.num.lattice.execute_bc(self.id)
+
+
+_SLICES = {"X0": X0, "XE": XE, "Y0": Y0, "YE": YE, "Z0": Z0, "ZE": ZE}
+
+
+def _faces(sl):
+    m = 0
+    for s in sl:
+        m |= _SLICES[s] if isinstance(s, str) else int(s)
+    return m
+
+
+def generateBCNoSlip(lbgk, sl_or_map):
+    if isinstance(sl_or_map, np.ndarray):
+        return _BC(lbgk, lbgk.lattice.add_map_bc(BC_NOSLIP_MAP, sl_or_map))
+    return _BC(lbgk, lbgk.lattice.add_slice_bc(BC_NOSLIP, _faces(sl_or_map)))
+
+
+def generateBCConstantVelocity(lbgk, v, sl):
+    return _BC(lbgk, lbgk.lattice.add_slice_bc(BC_CONSTANT_VELOCITY, _faces(sl), u=v))
+
+
+def generateBCConstantPressure(lbgk, p, sl_or_map):
+    if isinstance(sl_or_map, np.ndarray):
+        return _BC(lbgk, lbgk.lattice.add_map_bc(BC_PRESSURE_VELOCITY_MAP, sl_or_map, p=p))
+    return _BC(lbgk, lbgk.lattice.add_slice_bc(BC_CONSTANT_PRESSURE, _faces(sl_or_map), p=p))
+
+
+def generateBCConstantPressureVelocity(lbgk, p, v, sl_or_map):
+    if isinstance(sl_or_map, np.ndarray):
+        return _BC(lbgk, lbgk.lattice.add_map_bc(BC_PRESSURE_VELOCITY_MAP, sl_or_map, p=p, u=v))
+    return _BC(lbgk, lbgk.lattice.add_slice_bc(BC_CONSTANT_PRESSURE_VELOCITY, _faces(sl_or_map), p=p, u=v))
+
+
+def generateBCVelocity(lbgk, v, mapv, computationalDomain=None, rotation=None):
+    return _BC(lbgk, lbgk.lattice.add_map_bc(BC_VELOCITY_MAP, mapv, cdomain=computationalDomain, u=v, rot=rotation))

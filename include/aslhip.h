@@ -1,0 +1,193 @@
+/*
+ * aslhip.h — C ABI of the MI355X (gfx950) LBGK library.
+ *
+ * This is the drop-in boundary for ASL's lattice Boltzmann hot path. ASL drives
+ * that path through C++ objects (asl::LBGK, asl::LBGKUtilities and the LBGK
+ * boundary-condition classes) whose device work is OpenCL generated at run time.
+ * Here every such object maps to calls on one opaque lattice handle; the C++
+ * compatibility layer (asl_amd/compat) and the Python host mirror (asl_amd/) are
+ * both thin clients of these entry points.  Each declaration names the reference
+ * interface it replaces (paths are relative to the ASL source tree).
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a negative ASLHIP_E* code on
+ *     failure; aslhip_last_error() then returns the message (thread local).
+ *     ASL throws std::logic_error("ASL ERROR: ...") for the same conditions
+ *     (src/aslUtilities.cxx:68-95); the C++ layer rethrows that way.
+ *   - Host arrays use ASL's layout: the ghost-inclusive block of
+ *     (nx+2)*(ny+2)*(nz+2) cells, linear index x*Sy*Sz + y*Sz + z (last index
+ *     fastest, src/data/aslBlocks.h:118-129,189-196), one array per component
+ *     (component-major: u is ux[len] then uy[len] then uz[len]; f is f0[len] ...
+ *     f(Q-1)[len] in ASL's direction order, src/math/aslTemplates.cxx:146-200).
+ *   - Populations are ASL's weight-normalised f_i = F_i / w_i
+ *     (src/num/aslLBGK.cxx:46-72).
+ *   - Element type of host buffers: float for ASLHIP_F32 / ASLHIP_BF16 /
+ *     ASLHIP_F16 lattices, double for ASLHIP_F64 lattices.
+ *   - All work of a lattice runs in order on its own HIP stream.  Host reads
+ *     synchronise first, so what the host sees is synchronous like ASL's
+ *     blocking queue (src/acl/Kernels/aclKernel.cxx:289-313).
+ */
+#ifndef ASLHIP_H
+#define ASLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define ASLHIP_OK 0
+#define ASLHIP_EINVAL (-1)    /* bad argument                              */
+#define ASLHIP_ESTATE (-2)    /* call not allowed in the handle's state    */
+#define ASLHIP_EHIP (-3)      /* HIP runtime error                         */
+#define ASLHIP_ENOMEM (-4)    /* allocation failed                         */
+#define ASLHIP_EUNDEF (-5)    /* boundary conditions leave a link undefined */
+
+/* ---- storage / arithmetic type of the populations -------------------- */
+#define ASLHIP_F32 0  /* FP32 storage, FP32 collision                      */
+#define ASLHIP_F64 1  /* FP64 storage and collision (the FP64 GPU oracle)  */
+#define ASLHIP_BF16 2 /* BF16 storage of f_i - 1, FP32 collision           */
+#define ASLHIP_F16 3  /* FP16 storage of f_i - 1, FP32 collision           */
+
+/* ---- lattice flags (aslhip_lattice_create) --------------------------- */
+#define ASLHIP_PERIODIC_X 1u
+#define ASLHIP_PERIODIC_Y 2u
+#define ASLHIP_PERIODIC_Z 4u
+
+/* ---- faces: bit (1 << asl::SlicesNames), src/num/aslBCond.h:309 ------ */
+#define ASLHIP_X0 1u
+#define ASLHIP_XE 2u
+#define ASLHIP_Y0 4u
+#define ASLHIP_YE 8u
+#define ASLHIP_Z0 16u
+#define ASLHIP_ZE 32u
+
+/* ---- boundary-condition kinds ---------------------------------------- */
+/* slice BCs: src/num/aslLBGKBC.cxx:91-202, factories :745-798            */
+#define ASLHIP_BC_NOSLIP 1                   /* f_k(g) = f_kbar(g+c_k)           */
+#define ASLHIP_BC_CONSTANT_PRESSURE 2        /* f_k(g) = p                       */
+#define ASLHIP_BC_CONSTANT_VELOCITY 3        /* f_k(g) = f_kbar(g+c_k) + 6 u.c_k */
+#define ASLHIP_BC_CONSTANT_PRESSURE_VELOCITY 4 /* f_k(g) = p + 3 u.c_k          */
+/* map (level-set) BCs: src/num/aslLBGKBC.cxx:219-432, factories :801-875 */
+#define ASLHIP_BC_NOSLIP_MAP 5               /* f_i(g) = f_ibar(g+c_i)           */
+#define ASLHIP_BC_PRESSURE_VELOCITY_MAP 6    /* f_i(g) = feq(p, u)               */
+#define ASLHIP_BC_VELOCITY_MAP 7             /* f_i(g) = rho_g + 3 rho_g u.c_i   */
+
+/* ---- step flags ------------------------------------------------------ */
+#define ASLHIP_STEP_WRITE_MACRO 1u /* last step of the call writes rho and u */
+
+typedef struct aslhip_lattice aslhip_lattice;
+
+/* Message of the last failed call on this thread (empty if none).  Returns
+ * the message length. */
+int aslhip_last_error(char *buf, size_t n);
+
+/* Library version and the device it runs on (name written to buf). */
+int aslhip_device_info(char *buf, size_t n, int *cu_count, double *hbm_bytes);
+
+/* asl::LBGK::LBGK(Block, Param nu, const VectorTemplate*, bool compVel,
+ * bool compRho, CommandQueue)  — src/num/aslLBGK.h:112-116,
+ * src/num/aslLBGK.cxx:114-158.  nx, ny, nz: internal block size; q: 15, 19
+ * or 27 (d3q15()/d3q19()/d3q27(), src/math/aslTemplates.cxx:146-200);
+ * dtype: ASLHIP_F32/F64/BF16/F16; flags: ASLHIP_PERIODIC_* (periodic wrap is
+ * an extension: ASL has none). */
+int aslhip_lattice_create(aslhip_lattice **out, int nx, int ny, int nz, int q,
+                          int dtype, unsigned flags);
+
+/* Releases all device memory of the handle. */
+int aslhip_lattice_destroy(aslhip_lattice *h);
+
+/* Viscosity nu in lattice units; relaxation rate w = 1/(3 nu + 1/2) is
+ * evaluated in the lattice's arithmetic type like ASL's generated kernel
+ * (src/num/aslLBGK.cxx:203). */
+int aslhip_set_viscosity(aslhip_lattice *h, double nu);
+
+/* asl::LBGK::setCompressible — src/num/aslLBGK.h:129, equilibrium variants
+ * at src/num/aslLBGK.cxx:220-231. */
+int aslhip_set_compressible(aslhip_lattice *h, int compressible);
+
+/* compRho / compVel constructor flags (src/num/aslLBGK.cxx:210-218). */
+int aslhip_set_compute_macro(aslhip_lattice *h, int comp_rho, int comp_vel);
+
+/* asl::LBGKUtilities::initF(rho, vel) with constants —
+ * src/num/aslLBGK.cxx:262-280.  Initialises every cell of the ghost-inclusive
+ * block and writes rho and u. */
+int aslhip_init_equilibrium(aslhip_lattice *h, double rho, const double u[3]);
+
+/* asl::LBGKUtilities::initF(rho, vel) with per-cell fields (host arrays in
+ * ASL layout; rho may be NULL meaning 1). */
+int aslhip_init_equilibrium_field(aslhip_lattice *h, const void *rho,
+                                  const void *u);
+
+/* Slice BCs: asl::generateBCNoSlip / generateBCConstantPressure /
+ * generateBCConstantVelocity / generateBCConstantPressureVelocity(lbgk, ...,
+ * {slices}) — src/num/aslLBGKBC.h:281-300, src/num/aslLBGKBC.cxx:745-798,
+ * point lists src/num/aslBCond.cxx:373-510.  faces: ASLHIP_X0 | ... ;
+ * p and u as the factory's arguments (ignored where the kind has none). */
+int aslhip_bc_add_slice(aslhip_lattice *h, int kind, unsigned faces, double p,
+                        const double u[3], int *bc_id);
+
+/* Map BCs: asl::generateBCNoSlip(lbgk, map), generateBCConstantPressure(lbgk,
+ * p, map), generateBCConstantPressureVelocity(lbgk, p, u, map),
+ * generateBCVelocity(lbgk, v, map[, computationalDomain]) —
+ * src/num/aslLBGKBC.h:302-321, src/num/aslLBGKBC.cxx:801-875.  map and
+ * cdomain: host float arrays of the ghost-inclusive block (cdomain may be
+ * NULL).  For ASLHIP_BC_VELOCITY_MAP the wall velocity at position r is
+ * u + omega x (r - centre) (asl::generatePFConstant / PFRotationField,
+ * src/math/aslPositionFunction.cxx); rot = {omega[3], centre[3]} or NULL. */
+int aslhip_bc_add_map(aslhip_lattice *h, int kind, const float *map,
+                      const float *cdomain, double p, const double u[3],
+                      const double rot[6], int *bc_id);
+
+/* BC execute(): asl::BCLBGKCommon::execute / BCNoSlipMap::execute ...
+ * (src/num/aslLBGKBC.cxx:80-83,257-260).  The fused step applies every BC
+ * itself; this call records the driver's execute order, which decides who
+ * wins where two BCs fill the same ghost link (the last one executed). */
+int aslhip_bc_execute(aslhip_lattice *h, int bc_id);
+
+/* Builds the packed cell-type map and the link-override tables from the BCs
+ * and the current execute order.  Called implicitly by aslhip_step. */
+int aslhip_finalize(aslhip_lattice *h);
+
+/* asl::LBGK::execute() (preProcessing + collide, src/num/aslSingleKernelNM.cxx:
+ * 48-53) followed by the BCs, nsteps times.  flags: ASLHIP_STEP_*.  Runs
+ * asynchronously on the lattice's stream. */
+int aslhip_step(aslhip_lattice *h, int nsteps, unsigned flags);
+
+/* Blocks until the lattice's stream is idle. */
+int aslhip_synchronize(aslhip_lattice *h);
+
+/* asl::LBGK::getRho() / getVelocity() / getF() host views —
+ * src/num/aslLBGK.h:125-127.  out: len (rho), 3*len (u) or Q*len (f)
+ * elements of the host element type.  Cells that are not computation nodes
+ * hold unspecified values (ASL's hold collided ghost data). */
+int aslhip_read_rho(aslhip_lattice *h, void *out);
+int aslhip_read_velocity(aslhip_lattice *h, void *out);
+int aslhip_read_f(aslhip_lattice *h, void *out);
+
+/* Packed cell-type map (one byte per ghost-inclusive cell):
+ * 0 fluid, 1 fluid with BC links, 2 not computed (ghost/solid),
+ * 3 computed ghost.  Bits 4-7 unused. */
+int aslhip_read_cell_types(aslhip_lattice *h, uint8_t *out);
+
+/* FP64 reductions over computation nodes: out = {mass, mx, my, mz, KE}
+ * with mass = sum rho, m = sum rho u, KE = sum |m|^2 / (2 rho). */
+int aslhip_reduce_conservation(aslhip_lattice *h, double out[5]);
+
+/* Counts: computation nodes (counted in MLUPS), computed cells, boundary
+ * cells, distinct link patterns. */
+int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
+
+/* Times nsteps steps with HIP events recorded on the lattice's stream.
+ * ms_total: wall time of the whole call between the events; ms_kernel:
+ * summed time of the step kernels (events around each launch). */
+int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags,
+                      double *ms_total, double *ms_kernel);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASLHIP_H */

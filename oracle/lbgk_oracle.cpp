@@ -1,0 +1,716 @@
+// lbgk_oracle.cpp — CPU restatement of ASL's LBGK path (test infrastructure).
+//
+// Only tests/, the smoke check and bench.py's CPU-baseline leg load this
+// library.  It restates the reference algorithm; it is not the product.
+// Every function cites the ASL source it follows (paths relative to the ASL
+// tree).  Built with -ffp-contract=off so that every float/double operation
+// rounds exactly once, in the order written here.
+//
+// Operation order.  ASL's kernel is generated from expression trees; the order
+// of every sum and product below follows those trees
+// (src/acl/aclMath/aclVectorOfElementsOperations.cxx:336-364: a dot product is
+// a left-to-right sum a0*b0 + a1*b1 + ...).  Terms whose constant factor is an
+// exact zero (c_i components) are left out: for finite populations x + (+-0)
+// == x, so the value is unchanged.
+#include "lbgk_oracle.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Lattice templates — src/math/aslTemplates.cxx.
+// ---------------------------------------------------------------------------
+struct Templ {
+  int Q = 0;
+  int c[27][3];
+  double w[27];  // quasiparticlesCoefs: lc_i/6, +1 on the rest direction
+  int inv[27];
+};
+
+// VectorTemplate(int, AVec*, double* lc, double* gc): quasiparticlesCoefs[i] =
+// lc[i]/6, then [0] += 1 (src/math/aslTemplates.cxx:38-70); invertVectors by
+// pairwise search (:72-83).
+void finish_templ(Templ &t, const double *lc) {
+  for (int i = 0; i < t.Q; ++i) t.w[i] = lc[i] / 6.;
+  t.w[0] += 1.;
+  for (int i = 0; i < t.Q; ++i)
+    for (int j = i + 1; j < t.Q; ++j)
+      if (t.c[i][0] == -t.c[j][0] && t.c[i][1] == -t.c[j][1] && t.c[i][2] == -t.c[j][2]) {
+        t.inv[i] = j;
+        t.inv[j] = i;
+      }
+  t.inv[0] = 0;
+}
+
+Templ make_templ(int q) {
+  Templ t;
+  // d3q15(): src/math/aslTemplates.cxx:146-165
+  static const int c15[15][3] = {{0, 0, 0},  {1, 0, 0},   {0, 1, 0},   {0, 0, 1},   {-1, 0, 0},
+                                 {0, -1, 0}, {0, 0, -1},  {1, 1, 1},   {1, 1, -1},  {1, -1, 1},
+                                 {1, -1, -1}, {-1, 1, 1}, {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+  // d3q19() / first 19 of d3q27(): src/math/aslTemplates.cxx:168-200
+  static const int c27[27][3] = {
+      {0, 0, 0},   {1, 0, 0},   {0, 1, 0},  {0, 0, 1},   {-1, 0, 0},  {0, -1, 0},  {0, 0, -1},
+      {1, 1, 0},   {1, -1, 0},  {-1, -1, 0}, {-1, 1, 0}, {0, 1, 1},   {0, 1, -1},  {0, -1, -1},
+      {0, -1, 1},  {1, 0, 1},   {1, 0, -1}, {-1, 0, -1}, {-1, 0, 1},  {1, 1, 1},   {1, 1, -1},
+      {1, -1, 1},  {1, -1, -1}, {-1, 1, 1}, {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+  double lc[27];
+  if (q == 15) {
+    t.Q = 15;
+    std::memcpy(t.c, c15, sizeof(c15));
+    lc[0] = -14. / 3.;
+    for (int i = 1; i < 7; ++i) lc[i] = 2. / 3.;
+    for (int i = 7; i < 15; ++i) lc[i] = 1. / 12.;
+  } else if (q == 19) {
+    t.Q = 19;
+    std::memcpy(t.c, c27, sizeof(int) * 3 * 19);
+    lc[0] = -4.;
+    for (int i = 1; i < 7; ++i) lc[i] = 1. / 3.;
+    for (int i = 7; i < 19; ++i) lc[i] = 1. / 6.;
+  } else if (q == 27) {
+    // Extension: ASL builds d3q27() without weights.  Textbook D3Q27 weights
+    // 8/27, 2/27, 1/54, 1/216, written as lc = 6 w (rest: 6 (w0 - 1)).
+    t.Q = 27;
+    std::memcpy(t.c, c27, sizeof(c27));
+    lc[0] = 6. * (8. / 27. - 1.);
+    for (int i = 1; i < 7; ++i) lc[i] = 6. * (2. / 27.);
+    for (int i = 7; i < 19; ++i) lc[i] = 6. * (1. / 54.);
+    for (int i = 19; i < 27; ++i) lc[i] = 6. * (1. / 216.);
+  } else {
+    throw std::invalid_argument("lattice template must be 15, 19 or 27");
+  }
+  finish_templ(t, lc);
+  if (q == 27) {  // exact textbook values (the lc/6 round trip is not exact)
+    t.w[0] = 8. / 27.;
+    for (int i = 1; i < 7; ++i) t.w[i] = 2. / 27.;
+    for (int i = 7; i < 19; ++i) t.w[i] = 1. / 54.;
+    for (int i = 19; i < 27; ++i) t.w[i] = 1. / 216.;
+  }
+  return t;
+}
+
+enum Kind {
+  NOSLIP = 1,
+  CONSTANT_PRESSURE = 2,
+  CONSTANT_VELOCITY = 3,
+  CONSTANT_PRESSURE_VELOCITY = 4,
+  NOSLIP_MAP = 5,
+  PRESSURE_VELOCITY_MAP = 6,
+  VELOCITY_MAP = 7
+};
+
+template <typename T>
+struct BC {
+  int kind = 0;
+  unsigned faces = 0;
+  T p = 0;
+  T u[3] = {0, 0, 0};
+  bool has_rot = false;
+  T omega[3] = {0, 0, 0}, centre[3] = {0, 0, 0};
+  std::vector<float> map, cd;
+  // slice point list (BCond::addPoint, src/num/aslBCond.cxx:51-56)
+  std::vector<long> idx;
+  std::vector<int> dir;
+};
+
+template <typename T>
+struct Lattice {
+  Templ vt;
+  int N[3], S[3];
+  long Tr[3];  // c2iTransformVector of the ghost-inclusive block
+  long len;
+  bool periodic = false;
+  long pool_len = 0;
+  std::vector<T> pool;  // Q * pool_len
+  std::vector<long> shift, inc;
+  std::vector<T> fper, fper2;  // periodic mode: plain arrays Q * len
+  std::vector<T> rho, vel;
+  T nu = T(0.1);
+  bool compressible = true, comp_rho = true, comp_vel = true;
+  double dx = 1., pos0[3] = {0., 0., 0.};
+  std::vector<BC<T>> bcs;
+  // per-direction constants as ASL prints them (double -> T)
+  T cw[27];       // w_i
+  T cwc[27][3];   // c_i[d] * w_i
+
+  long c2i(int x, int y, int z) const { return x * Tr[0] + y * Tr[1] + z * Tr[2]; }
+  long off(int i) const { return c2i(vt.c[i][0], vt.c[i][1], vt.c[i][2]); }
+  T *win(int i) { return periodic ? &fper[(size_t)i * len] : &pool[(size_t)i * pool_len + shift[i]]; }
+
+  Lattice(int nx, int ny, int nz, int q, unsigned per) : vt(make_templ(q)) {
+    if (nx < 1 || ny < 1 || nz < 1) throw std::invalid_argument("block size must be positive");
+    if (per != 0 && per != 7) throw std::invalid_argument("periodic wrap must be on all three axes");
+    periodic = per == 7;
+    N[0] = nx; N[1] = ny; N[2] = nz;
+    for (int d = 0; d < 3; ++d) S[d] = N[d] + 2;  // offset(b, 1)
+    // castTransformVector (src/data/aslBlocks.h:118-129): last index fastest
+    Tr[2] = 1; Tr[1] = S[2]; Tr[0] = (long)S[1] * S[2];
+    len = (long)S[0] * S[1] * S[2];
+    for (int i = 0; i < vt.Q; ++i) {
+      cw[i] = T(vt.w[i]);
+      for (int d = 0; d < 3; ++d) cwc[i][d] = T(double(vt.c[i][d]) * vt.w[i]);
+    }
+    const T qnan = std::numeric_limits<T>::quiet_NaN();
+    if (periodic) {
+      fper.assign((size_t)vt.Q * len, qnan);
+      fper2.assign((size_t)vt.Q * len, qnan);
+    } else {
+      // LBGK::createData (src/num/aslLBGK.cxx:129-158)
+      shift.resize(vt.Q);
+      inc.resize(vt.Q);
+      long max_inc = 0;
+      for (int i = 0; i < vt.Q; ++i) {
+        inc[i] = -off(i);
+        if (inc[i] > max_inc) max_inc = inc[i];
+      }
+      pool_len = 2 * len + 4 * max_inc;
+      for (int i = 0; i < vt.Q; ++i) shift[i] = inc[i] > 0 ? 0 : pool_len - len;
+      // cl::Buffer memory is never cleared: stale memory is NaN here.
+      pool.assign((size_t)vt.Q * pool_len, qnan);
+    }
+    rho.assign(len, qnan);
+    vel.assign((size_t)3 * len, qnan);
+  }
+
+  // LBGK::preProcessing + createCopyKernels (src/num/aslLBGK.cxx:160-185,
+  // 237-250).  The window of direction i slides by inc_i; when it would leave
+  // the pool it is first copied to the pool start (inc > 0) or end.
+  void pre_processing() {
+    for (int i = 0; i < vt.Q; ++i) {
+      if (shift[i] + len + inc[i] > pool_len || shift[i] + inc[i] < 0) {
+        long dst = inc[i] > 0 ? 0 : pool_len - len;
+        T *base = &pool[(size_t)i * pool_len];
+        std::memmove(base + dst, base + shift[i], sizeof(T) * len);  // regions never overlap
+        shift[i] = dst;
+      }
+    }
+    for (int i = 0; i < vt.Q; ++i) shift[i] += inc[i];
+  }
+
+  // The collision of one cell, LBGK::init0 (src/num/aslLBGK.cxx:187-235).
+  inline void collide(T *f, T w, T &rho_o, T *v_o) const {
+    const int Q = vt.Q;
+    // momentum = computeMomentum(fOld) (:51-63), pRho = computeRho(fOld) (:46-49)
+    T m[3];
+    for (int d = 0; d < 3; ++d) {
+      bool first = true;
+      T acc = 0;
+      for (int j = 0; j < Q; ++j) {
+        if (vt.c[j][d] == 0) continue;
+        T t = f[j] * cwc[j][d];
+        acc = first ? t : acc + t;
+        first = false;
+      }
+      m[d] = acc;
+    }
+    T r = f[0] * cw[0];
+    for (int j = 1; j < Q; ++j) r = r + f[j] * cw[j];
+    rho_o = r;
+    if (compressible)
+      for (int d = 0; d < 3; ++d) v_o[d] = m[d] / r;
+    else
+      for (int d = 0; d < 3; ++d) v_o[d] = m[d];
+    // 1.5 * momentum * momentum (dot product), then / pRho (compressible)
+    T mm = (T(1.5) * m[0]) * m[0];
+    mm = mm + (T(1.5) * m[1]) * m[1];
+    mm = mm + (T(1.5) * m[2]) * m[2];
+    const T one_w = T(1.) - w;
+    const T mmr = compressible ? mm / r : mm;
+    for (int i = 0; i < Q; ++i) {
+      // aDotMomentum = momentum . c_i (a_i_dot_v, :65-72)
+      bool first = true;
+      T a = 0;
+      for (int d = 0; d < 3; ++d) {
+        if (vt.c[i][d] == 0) continue;
+        T t = vt.c[i][d] > 0 ? m[d] : -m[d];
+        a = first ? t : a + t;
+        first = false;
+      }
+      T feq;
+      if (i == 0) {
+        feq = r - mmr;  // rho + 3*0 + 4.5*0/rho - mm/rho
+      } else if (compressible) {
+        feq = ((r + T(3.) * a) + (T(4.5) * (a * a)) / r) - mmr;
+      } else {
+        feq = ((r + T(3.) * a) + T(4.5) * (a * a)) - mmr;
+      }
+      f[i] = f[i] * one_w + w * feq;  // fOld*(1-w) + w*fEq
+    }
+  }
+
+  void collide_all() {
+    const int Q = vt.Q;
+    const T w = T(1.) / (T(3.) * nu + T(.5));  // :203
+    if (!periodic) {
+      T *wp[27];
+      for (int i = 0; i < Q; ++i) wp[i] = win(i);
+#pragma omp parallel for schedule(static)
+      for (long j = 0; j < len; ++j) {
+        T f[27], r, v[3];
+        for (int i = 0; i < Q; ++i) f[i] = wp[i][j];
+        collide(f, w, r, v);
+        if (comp_rho) rho[j] = r;
+        if (comp_vel)
+          for (int d = 0; d < 3; ++d) vel[(size_t)d * len + j] = v[d];
+        for (int i = 0; i < Q; ++i) wp[i][j] = f[i];
+      }
+    } else {
+      // Extension: periodic pull streaming over the interior cells.
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int x = 1; x <= N[0]; ++x)
+        for (int y = 1; y <= N[1]; ++y)
+          for (int z = 1; z <= N[2]; ++z) {
+            long j = c2i(x, y, z);
+            T f[27], r, v[3];
+            for (int i = 0; i < Q; ++i) {
+              int xs = x - vt.c[i][0], ys = y - vt.c[i][1], zs = z - vt.c[i][2];
+              xs = xs < 1 ? N[0] : (xs > N[0] ? 1 : xs);
+              ys = ys < 1 ? N[1] : (ys > N[1] ? 1 : ys);
+              zs = zs < 1 ? N[2] : (zs > N[2] ? 1 : zs);
+              f[i] = fper[(size_t)i * len + c2i(xs, ys, zs)];
+            }
+            collide(f, w, r, v);
+            if (comp_rho) rho[j] = r;
+            if (comp_vel)
+              for (int d = 0; d < 3; ++d) vel[(size_t)d * len + j] = v[d];
+            for (int i = 0; i < Q; ++i) fper2[(size_t)i * len + j] = f[i];
+          }
+      fper.swap(fper2);
+    }
+  }
+
+  // SingleKernelNM::execute = preProcessing + compute (src/num/aslSingleKernelNM.cxx:48-53)
+  void lbgk_execute() {
+    if (!periodic) pre_processing();
+    collide_all();
+  }
+
+  // LBGKUtilities::initF(rho, vel) (src/num/aslLBGK.cxx:262-275):
+  // f_i = rho * (1 + 3 a_i + 4.5 a_i^2 - 1.5 u.u), a_i = u . c_i
+  void init_cell(long j, T r, const T *u) {
+    T uu = (T(1.5) * u[0]) * u[0];
+    uu = uu + (T(1.5) * u[1]) * u[1];
+    uu = uu + (T(1.5) * u[2]) * u[2];
+    for (int i = 0; i < vt.Q; ++i) {
+      T a = dot_c(u, i);
+      T fi = ((T(1.) + T(3.) * a) + T(4.5) * (a * a)) - uu;
+      win(i)[j] = r * fi;
+    }
+    rho[j] = r;
+    for (int d = 0; d < 3; ++d) vel[(size_t)d * len + j] = u[d];
+  }
+
+  // v . c_i with the zero terms left out (u0*c0 + u1*c1 + u2*c2)
+  T dot_c(const T *u, int i) const {
+    bool first = true;
+    T a = 0;
+    for (int d = 0; d < 3; ++d) {
+      if (vt.c[i][d] == 0) continue;
+      T t = vt.c[i][d] > 0 ? u[d] : -u[d];
+      a = first ? t : a + t;
+      first = false;
+    }
+    return a;
+  }
+
+  // ---------------------------------------------------------------------
+  // Boundary conditions
+  // ---------------------------------------------------------------------
+  // addSliceX0 ... addSliceZE (src/num/aslBCond.cxx:373-493): for the face
+  // with inward direction e = vectors[1..6] and each k with c_k . e > 0, every
+  // face-layer node p (tangential indices 1..S-2) gives the entry
+  // (p - c_k, k).
+  void add_slices(BC<T> &bc) {
+    const int Q = vt.Q;
+    static const int face_dir[6] = {1, 4, 2, 5, 3, 6};  // X0 XE Y0 YE Z0 ZE
+    for (int f = 0; f < 6; ++f) {
+      if (!(bc.faces & (1u << f))) continue;
+      const int *e = vt.c[face_dir[f]];
+      int axis = f / 2;
+      int layer = (f % 2 == 0) ? 1 : S[axis] - 2;
+      int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+      for (int k = 1; k < Q; ++k) {
+        int dotp = vt.c[k][0] * e[0] + vt.c[k][1] * e[1] + vt.c[k][2] * e[2];
+        if (dotp <= 0) continue;
+        for (int i = 1; i < S[a1] - 1; ++i)
+          for (int j = 1; j < S[a2] - 1; ++j) {
+            int p[3];
+            p[axis] = layer;
+            p[a1] = i;
+            p[a2] = j;
+            bc.idx.push_back(c2i(p[0] - vt.c[k][0], p[1] - vt.c[k][1], p[2] - vt.c[k][2]));
+            bc.dir.push_back(k);
+          }
+      }
+    }
+  }
+
+  // Maps are float fields (the drivers' FlT); their compares are made in float.
+  float map_safe(const std::vector<float> &m, long idx, float out) const {
+    // generateVEOutOfBoundarySafe(a, outVal) (src/acl/aclMath/aclMathAlg.cxx:84-94)
+    return (idx < 0 || idx > len - 1) ? out : m[(size_t)idx];
+  }
+  T f_safe(int i, long idx) {
+    // generateVEOutOfBoundarySafe(a) (src/acl/aclMath/aclMathAlg.cxx:76-81)
+    return win(i)[(len + idx) % len];
+  }
+  bool comp_node(const BC<T> &bc, long idx) const {
+    // isComputationNode (src/num/aslBCond.cxx:204-211;
+    // src/math/aslDistanceFunctionAlg.cxx:45-48): map > 0 [&& cdomain > 0]
+    bool r = map_safe(bc.map, idx, -.9f) > 0.f;
+    if (!bc.cd.empty()) r = r && map_safe(bc.cd, idx, -1.f) > 0.f;
+    return r;
+  }
+
+  void execute_bc(BC<T> &bc) {
+    const int Q = vt.Q;
+    switch (bc.kind) {
+      case NOSLIP:  // BCNoSlip::init (src/num/aslLBGKBC.cxx:91-113)
+        for (size_t e = 0; e < bc.idx.size(); ++e) {
+          int k = bc.dir[e];
+          long g = bc.idx[e];
+          win(k)[g] = win(vt.inv[k])[g + off(k)];
+        }
+        break;
+      case CONSTANT_PRESSURE:  // BCConstantPressure::init (:121-142)
+        for (size_t e = 0; e < bc.idx.size(); ++e) win(bc.dir[e])[bc.idx[e]] = bc.p;
+        break;
+      case CONSTANT_VELOCITY:  // BCConstantVelocity::init (:149-172)
+        for (size_t e = 0; e < bc.idx.size(); ++e) {
+          int k = bc.dir[e];
+          long g = bc.idx[e];
+          T add = T(6.) * dot_c(bc.u, k);
+          win(k)[g] = win(vt.inv[k])[g + off(k)] + add;
+        }
+        break;
+      case CONSTANT_PRESSURE_VELOCITY:  // BCConstantPressureVelocity::init (:182-202)
+        for (size_t e = 0; e < bc.idx.size(); ++e) {
+          int k = bc.dir[e];
+          win(k)[bc.idx[e]] = bc.p + T(3.) * dot_c(bc.u, k);
+        }
+        break;
+      case NOSLIP_MAP:  // BCNoSlipMap::init (:219-254)
+#pragma omp parallel for schedule(static)
+        for (long j = 0; j < len; ++j) {
+          const float mj = bc.map[(size_t)j];
+          if (!(mj <= 0.f && mj > -.9999f)) continue;
+          for (int i = 1; i < Q; ++i)
+            if (comp_node(bc, j + off(i))) win(i)[j] = f_safe(vt.inv[i], j + off(i));
+        }
+        break;
+      case PRESSURE_VELOCITY_MAP: {  // BCConstantPressureVelocityMap::init (:386-432)
+        T feq[27];
+        const T *u = bc.u;
+        const T p = bc.p;
+        T uu = u[0] * u[0];
+        uu = uu + u[1] * u[1];
+        uu = uu + u[2] * u[2];
+        for (int i = 1; i < Q; ++i) {
+          T a = dot_c(u, i);
+          if (compressible)
+            feq[i] = ((p + (T(3.) * p) * a) + ((T(4.5) * p) * a) * a) - (T(1.5) * p) * uu;
+          else {
+            // 3.*velocity*c_i is the dot product of (3u) with c_i
+            T u3[3] = {T(3.) * u[0], T(3.) * u[1], T(3.) * u[2]};
+            feq[i] = ((p + dot_c(u3, i)) + (T(4.5) * a) * a) - T(1.5) * uu;
+          }
+        }
+#pragma omp parallel for schedule(static)
+        for (long j = 0; j < len; ++j) {
+          const float mj = bc.map[(size_t)j];
+          if (!(mj <= 0.f && mj > -.9999f)) continue;
+          for (int i = 1; i < Q; ++i)
+            if (comp_node(bc, j + off(i))) win(i)[j] = feq[i];
+        }
+        break;
+      }
+      case VELOCITY_MAP: {  // BCVelocityMap::init (:290-348)
+#pragma omp parallel for schedule(static)
+        for (long j = 0; j < len; ++j) {
+          const float mj = bc.map[(size_t)j];
+          if (!(mj <= 0.f && mj > -.9999f)) continue;
+          T coef = T(0), r = T(0);
+          for (int i = 1; i < Q; ++i) {
+            if (!comp_node(bc, j + off(i))) continue;
+            coef = coef + cw[i];
+            r = r + f_safe(vt.inv[i], j + off(i)) * cw[i];
+          }
+          r = r / coef;
+          // Index2PositionACL over the ghost-inclusive block
+          // (src/math/aslIndex2Position.cxx:64-108)
+          long I[3] = {j / Tr[0], (j % Tr[0]) / Tr[1], (j % Tr[0]) % Tr[1]};
+          T pos[3];
+          for (int d = 0; d < 3; ++d) pos[d] = T(pos0[d] - dx) + T(dx) * T(I[d]);
+          T uw[3] = {bc.u[0], bc.u[1], bc.u[2]};
+          if (bc.has_rot) {
+            // PFRotationField: crossProduct(axis, pos - c)
+            // (src/math/aslPositionFunction.cxx:180-190; cross product
+            // src/acl/aclMath/aclVectorOfElementsOperations.cxx:514-535)
+            T d0 = pos[0] - bc.centre[0], d1 = pos[1] - bc.centre[1], d2 = pos[2] - bc.centre[2];
+            uw[0] = bc.omega[1] * d2 - bc.omega[2] * d1;
+            uw[1] = bc.omega[2] * d0 - bc.omega[0] * d2;
+            uw[2] = bc.omega[0] * d1 - bc.omega[1] * d0;
+          }
+          T mom[3] = {uw[0] * r, uw[1] * r, uw[2] * r};
+          for (int i = 1; i < Q; ++i)
+            if (comp_node(bc, j + off(i))) win(i)[j] = r + dot_c(mom, i) * T(3.);
+        }
+        break;
+      }
+      default:
+        throw std::invalid_argument("unknown boundary-condition kind");
+    }
+  }
+};
+
+struct Handle {
+  bool fp64;
+  std::unique_ptr<Lattice<float>> f;
+  std::unique_ptr<Lattice<double>> d;
+};
+
+template <typename F>
+auto dispatch(void *h, F fn) {
+  Handle *H = static_cast<Handle *>(h);
+  if (H->fp64) return fn(*H->d);
+  return fn(*H->f);
+}
+
+template <typename T>
+int add_slice(Lattice<T> &L, int kind, unsigned faces, double p, const double *u) {
+  if (L.periodic) throw std::invalid_argument("slice BCs need a non-periodic lattice");
+  if (kind < NOSLIP || kind > CONSTANT_PRESSURE_VELOCITY) throw std::invalid_argument("not a slice BC kind");
+  BC<T> bc;
+  bc.kind = kind;
+  bc.faces = faces;
+  bc.p = T(p);
+  for (int d = 0; d < 3; ++d) bc.u[d] = u ? T(u[d]) : T(0);
+  L.add_slices(bc);
+  L.bcs.push_back(std::move(bc));
+  return (int)L.bcs.size() - 1;
+}
+
+template <typename T>
+int add_map(Lattice<T> &L, int kind, const float *map, const float *cd, double p, const double *u,
+            const double *rot) {
+  if (L.periodic) throw std::invalid_argument("map BCs need a non-periodic lattice");
+  if (kind < NOSLIP_MAP || kind > VELOCITY_MAP) throw std::invalid_argument("not a map BC kind");
+  BC<T> bc;
+  bc.kind = kind;
+  bc.p = T(p);
+  for (int d = 0; d < 3; ++d) bc.u[d] = u ? T(u[d]) : T(0);
+  bc.map.assign(map, map + L.len);
+  if (cd) bc.cd.assign(cd, cd + L.len);
+  if (rot) {
+    bc.has_rot = true;
+    for (int d = 0; d < 3; ++d) {
+      bc.omega[d] = T(rot[d]);
+      bc.centre[d] = T(rot[3 + d]);
+    }
+  }
+  L.bcs.push_back(std::move(bc));
+  return (int)L.bcs.size() - 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+void *orc_create(int nx, int ny, int nz, int q, int fp64, unsigned periodic) {
+  try {
+    Handle *h = new Handle();
+    h->fp64 = fp64 != 0;
+    if (h->fp64)
+      h->d.reset(new Lattice<double>(nx, ny, nz, q, periodic));
+    else
+      h->f.reset(new Lattice<float>(nx, ny, nz, q, periodic));
+    return h;
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+void orc_destroy(void *h) { delete static_cast<Handle *>(h); }
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+void orc_set_viscosity(void *h, double nu) {
+  dispatch(h, [&](auto &L) { L.nu = decltype(L.nu)(nu); });
+}
+void orc_set_compressible(void *h, int c) {
+  dispatch(h, [&](auto &L) { L.compressible = c != 0; });
+}
+void orc_set_compute_macro(void *h, int r, int v) {
+  dispatch(h, [&](auto &L) {
+    L.comp_rho = r != 0;
+    L.comp_vel = v != 0;
+  });
+}
+void orc_set_geometry(void *h, double dx, const double pos[3]) {
+  dispatch(h, [&](auto &L) {
+    L.dx = dx;
+    for (int d = 0; d < 3; ++d) L.pos0[d] = pos[d];
+  });
+}
+
+void orc_init_equilibrium(void *h, double rho, const double u[3]) {
+  dispatch(h, [&](auto &L) {
+    using T = decltype(L.nu);
+    T uu[3] = {T(u[0]), T(u[1]), T(u[2])};
+    for (long j = 0; j < L.len; ++j) L.init_cell(j, T(rho), uu);
+  });
+}
+
+void orc_init_equilibrium_field(void *h, const double *rho, const double *u) {
+  dispatch(h, [&](auto &L) {
+    using T = decltype(L.nu);
+    for (long j = 0; j < L.len; ++j) {
+      T uu[3] = {T(u[j]), T(u[L.len + j]), T(u[2 * L.len + j])};
+      L.init_cell(j, rho ? T(rho[j]) : T(1), uu);
+    }
+  });
+}
+
+int orc_bc_add_slice(void *h, int kind, unsigned faces, double p, const double u[3]) {
+  try {
+    return dispatch(h, [&](auto &L) { return add_slice(L, kind, faces, p, u); });
+  } catch (...) {
+    return -1;
+  }
+}
+
+int orc_bc_add_map(void *h, int kind, const float *map, const float *cd, double p, const double u[3],
+                   const double rot[6]) {
+  try {
+    return dispatch(h, [&](auto &L) { return add_map(L, kind, map, cd, p, u, rot); });
+  } catch (...) {
+    return -1;
+  }
+}
+
+int orc_bc_execute(void *h, int id) {
+  return dispatch(h, [&](auto &L) {
+    if (id < 0 || id >= (int)L.bcs.size()) return -1;
+    L.execute_bc(L.bcs[id]);
+    return 0;
+  });
+}
+
+void orc_lbgk_execute(void *h) {
+  dispatch(h, [&](auto &L) { L.lbgk_execute(); });
+}
+
+void orc_run(void *h, int nsteps, const int *order, int norder) {
+  dispatch(h, [&](auto &L) {
+    for (int s = 0; s < nsteps; ++s) {
+      L.lbgk_execute();
+      for (int b = 0; b < norder; ++b) L.execute_bc(L.bcs[order[b]]);
+    }
+  });
+}
+
+void orc_read_rho(void *h, double *out) {
+  dispatch(h, [&](auto &L) {
+    for (long j = 0; j < L.len; ++j) out[j] = double(L.rho[j]);
+  });
+}
+void orc_read_velocity(void *h, double *out) {
+  dispatch(h, [&](auto &L) {
+    for (long j = 0; j < 3 * L.len; ++j) out[j] = double(L.vel[j]);
+  });
+}
+void orc_read_f(void *h, double *out) {
+  dispatch(h, [&](auto &L) {
+    for (int i = 0; i < L.vt.Q; ++i) {
+      auto *w = L.win(i);
+      for (long j = 0; j < L.len; ++j) out[(size_t)i * L.len + j] = double(w[j]);
+    }
+  });
+}
+void orc_write_f(void *h, const double *in) {
+  dispatch(h, [&](auto &L) {
+    using T = decltype(L.nu);
+    for (int i = 0; i < L.vt.Q; ++i) {
+      auto *w = L.win(i);
+      for (long j = 0; j < L.len; ++j) w[j] = T(in[(size_t)i * L.len + j]);
+    }
+  });
+}
+
+long orc_slice_entries(void *h, int id) {
+  return dispatch(h, [&](auto &L) -> long {
+    if (id < 0 || id >= (int)L.bcs.size()) return -1;
+    return (long)L.bcs[id].idx.size();
+  });
+}
+
+// bit0: ghost node (map <= 0); bit1: the map guard (map <= 0 && map > -0.9999);
+// bit2: computation node (map > 0 [&& cdomain > 0]).
+void orc_map_masks(void *h, int id, uint8_t *out) {
+  dispatch(h, [&](auto &L) {
+    auto &bc = L.bcs[id];
+    for (long j = 0; j < L.len; ++j) {
+      const float m = bc.map[(size_t)j];
+      uint8_t v = 0;
+      if (m <= 0.f) v |= 1;
+      if (m <= 0.f && m > -.9999f) v |= 2;
+      if (L.comp_node(bc, j)) v |= 4;
+      out[j] = v;
+    }
+  });
+}
+
+// FP64 sums over cells with mask != 0 of the current populations:
+// {sum rho, sum m_x, sum m_y, sum m_z, sum |m|^2/(2 rho)}.
+void orc_conservation(void *h, const uint8_t *mask, double out[5]) {
+  dispatch(h, [&](auto &L) {
+    double s[5] = {0, 0, 0, 0, 0};
+    for (long j = 0; j < L.len; ++j) {
+      if (!mask[j]) continue;
+      double r = 0, m[3] = {0, 0, 0};
+      for (int i = 0; i < L.vt.Q; ++i) {
+        double fi = double(L.win(i)[j]) * L.vt.w[i];
+        r += fi;
+        for (int d = 0; d < 3; ++d) m[d] += fi * L.vt.c[i][d];
+      }
+      s[0] += r;
+      for (int d = 0; d < 3; ++d) s[1 + d] += m[d];
+      s[4] += (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) / (2. * r);
+    }
+    for (int k = 0; k < 5; ++k) out[k] = s[k];
+  });
+}
+
+int orc_template(int q, int *c, double *w, int *inv) {
+  try {
+    Templ t = make_templ(q);
+    for (int i = 0; i < t.Q; ++i) {
+      for (int d = 0; d < 3; ++d) c[3 * i + d] = t.c[i][d];
+      w[i] = t.w[i];
+      inv[i] = t.inv[i];
+    }
+    return t.Q;
+  } catch (...) {
+    return -1;
+  }
+}
+
+}  // extern "C"

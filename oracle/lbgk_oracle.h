@@ -1,0 +1,60 @@
+/*
+ * lbgk_oracle.h — CPU oracle for the LBGK hot path (test infrastructure only).
+ *
+ * A plain C++ restatement of ASL's asl::LBGK and its LBGK boundary
+ * conditions, used by tests/, by the smoke check and as bench.py's CPU
+ * baseline.  It is never part of the shipped GPU path.
+ *
+ * What it restates (paths relative to the ASL source tree):
+ *   - lattice templates d3q15/d3q19 (src/math/aslTemplates.cxx:30-83,146-185);
+ *     d3q27 weights are an extension (ASL's d3q27 has none, :187-200)
+ *   - pool/window streaming: createData, createCopyKernels, preProcessing
+ *     (src/num/aslLBGK.cxx:129-185,237-250), including the linear-index wrap
+ *     and the stale pool memory (filled with NaN here so that any dependence
+ *     on it shows up)
+ *   - the collision kernel LBGK::init0 (src/num/aslLBGK.cxx:187-235) over every
+ *     cell of the ghost-inclusive block, in ASL's operation order
+ *   - LBGKUtilities::initF (src/num/aslLBGK.cxx:262-280)
+ *   - slice BCs with their point lists (src/num/aslBCond.cxx:373-510,
+ *     src/num/aslLBGKBC.cxx:63-202) and map BCs with the out-of-block-safe
+ *     views and the map guard (src/num/aslLBGKBC.cxx:219-432,
+ *     src/aslGenerators.cxx:74-93, src/acl/aclMath/aclMathAlg.cxx:76-94)
+ *   - periodic wrap (extension; ASL has none) as plain pull streaming.
+ *
+ * All host I/O is double; arithmetic is float or double as chosen at create.
+ */
+#ifndef LBGK_ORACLE_H
+#define LBGK_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void *orc_create(int nx, int ny, int nz, int q, int fp64, unsigned periodic);
+void orc_destroy(void *h);
+int orc_set_threads(int n);
+void orc_set_viscosity(void *h, double nu);
+void orc_set_compressible(void *h, int c);
+void orc_set_compute_macro(void *h, int comp_rho, int comp_vel);
+void orc_set_geometry(void *h, double dx, const double pos[3]);
+void orc_init_equilibrium(void *h, double rho, const double u[3]);
+void orc_init_equilibrium_field(void *h, const double *rho, const double *u);
+int orc_bc_add_slice(void *h, int kind, unsigned faces, double p, const double u[3]);
+int orc_bc_add_map(void *h, int kind, const float *map, const float *cdomain,
+                   double p, const double u[3], const double rot[6]);
+int orc_bc_execute(void *h, int id);
+void orc_lbgk_execute(void *h);
+void orc_run(void *h, int nsteps, const int *order, int norder);
+void orc_read_rho(void *h, double *out);
+void orc_read_velocity(void *h, double *out);
+void orc_read_f(void *h, double *out);
+void orc_write_f(void *h, const double *in);
+long orc_slice_entries(void *h, int id);
+void orc_map_masks(void *h, int id, uint8_t *out);
+void orc_conservation(void *h, const uint8_t *mask, double out[5]);
+int orc_template(int q, int *c, double *w, int *inv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
