@@ -16,6 +16,7 @@ BC_NOSLIP, BC_CONSTANT_PRESSURE, BC_CONSTANT_VELOCITY, BC_CONSTANT_PRESSURE_VELO
 BC_NOSLIP_MAP, BC_PRESSURE_VELOCITY_MAP, BC_VELOCITY_MAP = 5, 6, 7
 STEP_WRITE_MACRO = 1
 CODE_FLUID, CODE_FLUID_BC, CODE_SKIP, CODE_GHOST = 0, 1, 2, 3
+TUNE_XCD_REMAP = 1
 
 # Every entry point of include/aslhip.h with its ctypes signature.
 _vp = ctypes.c_void_p
@@ -45,6 +46,7 @@ SIGNATURES = {
     "aslhip_read_cell_types": [_vp, _vp],
     "aslhip_reduce_conservation": [_vp, _dp],
     "aslhip_stats": [_vp, ctypes.POINTER(ctypes.c_int64)],
+    "aslhip_set_tuning": [_vp, ctypes.c_int, ctypes.c_int],
     "aslhip_time_steps": [_vp, ctypes.c_int, ctypes.c_uint, _dp, _dp],
 }
 

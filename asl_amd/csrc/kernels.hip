@@ -41,20 +41,22 @@ constexpr int BLOCK = 256;
 template <class S>
 struct Codec;
 
+// ld/st take a direction array base and a 32-bit byte offset, so loads use
+// the scalar-base + 32-bit vector-offset addressing form.
 template <>
 struct Codec<float> {
   using C = float;
   using M = float;
-  static __device__ __forceinline__ C ld(const float *p, int i) { return p[i]; }
-  static __device__ __forceinline__ void st(float *p, int i, C v) { p[i] = v; }
+  static __device__ __forceinline__ C ld(const char *b, unsigned o) { return *reinterpret_cast<const float *>(b + o); }
+  static __device__ __forceinline__ void st(char *b, unsigned o, C v) { *reinterpret_cast<float *>(b + o) = v; }
 };
 
 template <>
 struct Codec<double> {
   using C = double;
   using M = double;
-  static __device__ __forceinline__ C ld(const double *p, int i) { return p[i]; }
-  static __device__ __forceinline__ void st(double *p, int i, C v) { p[i] = v; }
+  static __device__ __forceinline__ C ld(const char *b, unsigned o) { return *reinterpret_cast<const double *>(b + o); }
+  static __device__ __forceinline__ void st(char *b, unsigned o, C v) { *reinterpret_cast<double *>(b + o) = v; }
 };
 
 // BF16 bit pattern of f - 1.
@@ -62,16 +64,16 @@ template <>
 struct Codec<unsigned short> {
   using C = float;
   using M = float;
-  static __device__ __forceinline__ C ld(const unsigned short *p, int i) {
-    return 1.0f + __uint_as_float(static_cast<unsigned>(p[i]) << 16);
+  static __device__ __forceinline__ C ld(const char *b, unsigned o) {
+    return 1.0f + __uint_as_float(static_cast<unsigned>(*reinterpret_cast<const unsigned short *>(b + o)) << 16);
   }
-  static __device__ __forceinline__ void st(unsigned short *p, int i, C v) {
+  static __device__ __forceinline__ void st(char *b, unsigned o, C v) {
     unsigned u = __float_as_uint(v - 1.0f);
     if ((u & 0x7fffffffu) > 0x7f800000u)
       u |= 0x00400000u;  // keep NaN quiet
     else
       u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-    p[i] = static_cast<unsigned short>(u >> 16);
+    *reinterpret_cast<unsigned short *>(b + o) = static_cast<unsigned short>(u >> 16);
   }
 };
 
@@ -79,9 +81,16 @@ template <>
 struct Codec<_Float16> {
   using C = float;
   using M = float;
-  static __device__ __forceinline__ C ld(const _Float16 *p, int i) { return 1.0f + static_cast<float>(p[i]); }
-  static __device__ __forceinline__ void st(_Float16 *p, int i, C v) { p[i] = static_cast<_Float16>(v - 1.0f); }
+  static __device__ __forceinline__ C ld(const char *b, unsigned o) {
+    return 1.0f + static_cast<float>(*reinterpret_cast<const _Float16 *>(b + o));
+  }
+  static __device__ __forceinline__ void st(char *b, unsigned o, C v) {
+    *reinterpret_cast<_Float16 *>(b + o) = static_cast<_Float16>(v - 1.0f);
+  }
 };
+
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 template <int Q>
 __device__ __forceinline__ constexpr int offc(int k, int sx, int sy) {
@@ -105,9 +114,14 @@ __device__ __forceinline__ C dot_c(const C *u, int i) {
 }
 
 // The BGK collision of one cell: ASL's LBGK::init0 (src/num/aslLBGK.cxx:
-// 187-235) with computeRho / computeMomentum / a_i_dot_v (:46-72).
+// 187-235) with computeRho / computeMomentum / a_i_dot_v (:46-72).  Each
+// x*y + z of ASL's expression tree is one fma (OpenCL's default FP_CONTRACT
+// ON); the CPU oracle contracts the same sites.  Opposite directions share
+// (4.5 a^2)/rho: a_inv(i) = -a_i exactly in IEEE arithmetic, so this is the
+// same value ASL computes twice.  On return v holds the post-collision
+// populations, r the density and m the momentum.
 template <int Q, class C>
-__device__ __forceinline__ void collide(C (&v)[Q], C w, bool compressible, C &r_o, C (&vel)[3]) {
+__device__ __forceinline__ void collide(C (&v)[Q], C w, bool compressible, C &r_o, C (&m_o)[3]) {
   using LT = Lat<Q>;
   C m[3];
 #pragma unroll
@@ -117,41 +131,42 @@ __device__ __forceinline__ void collide(C (&v)[Q], C w, bool compressible, C &r_
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
       if (LT::c[j][d] == 0) continue;
-      const C t = v[j] * C(double(LT::c[j][d]) * LT::w(j));
-      acc = first ? t : acc + t;
+      const C cw = C(double(LT::c[j][d]) * LT::w(j));
+      acc = first ? v[j] * cw : fmaT(v[j], cw, acc);
       first = false;
     }
     m[d] = acc;
   }
   C r = v[0] * C(LT::w(0));
 #pragma unroll
-  for (int j = 1; j < Q; ++j) r = r + v[j] * C(LT::w(j));
-  r_o = r;
+  for (int j = 1; j < Q; ++j) r = fmaT(v[j], C(LT::w(j)), r);
   C mm = (C(1.5) * m[0]) * m[0];
-  mm = mm + (C(1.5) * m[1]) * m[1];
-  mm = mm + (C(1.5) * m[2]) * m[2];
-  C mmr;
-  if (compressible) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) vel[d] = m[d] / r;
-    mmr = mm / r;
-  } else {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) vel[d] = m[d];
-    mmr = mm;
-  }
+  mm = fmaT(C(1.5) * m[1], m[1], mm);
+  mm = fmaT(C(1.5) * m[2], m[2], mm);
+  const C mmr = compressible ? mm / r : mm;
   const C one_w = C(1.) - w;
-  v[0] = v[0] * one_w + w * (r - mmr);
+  v[0] = fmaT(v[0], one_w, w * (r - mmr));
 #pragma unroll
   for (int i = 1; i < Q; ++i) {
+    const int j = inv_dir<Q>(i);
+    if (j < i) continue;
     const C a = dot_c<Q>(m, i);
-    C feq;
-    if (compressible)
-      feq = ((r + C(3.) * a) + (C(4.5) * (a * a)) / r) - mmr;
-    else
-      feq = ((r + C(3.) * a) + C(4.5) * (a * a)) - mmr;
-    v[i] = v[i] * one_w + w * feq;
+    C fi, fj;
+    if (compressible) {
+      const C t = (C(4.5) * (a * a)) / r;
+      fi = (fmaT(C(3.), a, r) + t) - mmr;
+      fj = (fmaT(C(3.), -a, r) + t) - mmr;
+    } else {
+      const C aa = a * a;
+      fi = fmaT(C(4.5), aa, fmaT(C(3.), a, r)) - mmr;
+      fj = fmaT(C(4.5), aa, fmaT(C(3.), -a, r)) - mmr;
+    }
+    v[i] = fmaT(v[i], one_w, w * fi);
+    v[j] = fmaT(v[j], one_w, w * fj);
   }
+  r_o = r;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) m_o[d] = m[d];
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (id % 8).  Give each
@@ -164,94 +179,164 @@ __device__ __forceinline__ int xcd_remap(int L, int n) {
 
 __device__ __forceinline__ int wrap1(int v, int n) { return v < 1 ? n : (v > n ? 1 : v); }
 
-template <int Q, class S, bool ODD, bool PER>
+template <class C>
+__device__ __forceinline__ C face_val(const StepArgs &a, int f, int k);
+template <>
+__device__ __forceinline__ float face_val<float>(const StepArgs &a, int f, int k) {
+  return a.fval32[f][k];
+}
+template <>
+__device__ __forceinline__ double face_val<double>(const StepArgs &a, int f, int k) {
+  return a.fval64[f][k];
+}
+
+template <int Q, class S, bool ODD, bool PER, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_step(const StepArgs a) {
   using CD = Codec<S>;
   using C = typename CD::C;
   using M = typename CD::M;
-  const int L = xcd_remap(blockIdx.x, a.nblocks);
+  constexpr unsigned ES = sizeof(S);
+  const int L = a.remap ? xcd_remap(blockIdx.x, a.nblocks) : static_cast<int>(blockIdx.x);
   const int plane = L / a.bpp;
   const int chunk = L - plane * a.bpp;
-  const int off = chunk * BLOCK + static_cast<int>(threadIdx.x);
-  if (off >= a.ny * a.sy) return;
-  const int idx = (plane + 1) * a.sx + a.sy + off;
-  const unsigned char code = a.code[idx];
-  if (code == CODE_SKIP) return;
+  const unsigned off = static_cast<unsigned>(chunk * BLOCK) + threadIdx.x;
+  if (off >= static_cast<unsigned>(a.ny * a.sy)) return;
+  // Row and column inside the plane (exact: off * sy < 2^40, see StepArgs::zmagic).
+  const unsigned row = static_cast<unsigned>((static_cast<unsigned long long>(off) * a.zmagic) >> 40);
+  const unsigned z = off - row * static_cast<unsigned>(a.sy);
+  if (z == 0 || z == static_cast<unsigned>(a.sy - 1)) return;  // ghost columns
+  const unsigned idx = static_cast<unsigned>((plane + 1) * a.sx + a.sy) + off;
+  const int x = plane + 1, y = static_cast<int>(row) + 1, zi = static_cast<int>(z);
+  const unsigned ib = idx * ES;
 
-  auto fk = [&](int k) -> S * { return reinterpret_cast<S *>(a.f + k * a.qs_bytes); };
-
-  // Neighbour indices (periodic lattices wrap over the interior).
-  int nbm[Q], nbp[Q];
-  if constexpr (PER) {
-    const int x = plane + 1, y = off / a.sy + 1, z = off - (y - 1) * a.sy;
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-      nbm[k] = wrap1(x - Lat<Q>::c[k][0], a.nx) * a.sx + wrap1(y - Lat<Q>::c[k][1], a.ny) * a.sy +
-               wrap1(z - Lat<Q>::c[k][2], a.nz);
-      nbp[k] = wrap1(x + Lat<Q>::c[k][0], a.nx) * a.sx + wrap1(y + Lat<Q>::c[k][1], a.ny) * a.sy +
-               wrap1(z + Lat<Q>::c[k][2], a.nz);
+  // Byte offsets of x - c_k (rd) and x + c_k (wr).
+  auto nb = [&](int k, int sgn) -> unsigned {
+    if constexpr (PER) {
+      return ES * static_cast<unsigned>(wrap1(x + sgn * Lat<Q>::c[k][0], a.nx) * a.sx +
+                                        wrap1(y + sgn * Lat<Q>::c[k][1], a.ny) * a.sy +
+                                        wrap1(zi + sgn * Lat<Q>::c[k][2], a.nz));
+    } else {
+      const unsigned o = ES * static_cast<unsigned>(offc<Q>(k, a.sx, a.sy));
+      return sgn > 0 ? ib + o : ib - o;
     }
-  } else {
+  };
+
+  // Face rule (see StepArgs): bit f of fb is set if the cell lies on face f.
+  const bool raw = (a.flags & STEP_F_RAW) != 0;
+  unsigned fb = 0;
+  if (!PER && !raw)
+    fb = (x == 1 ? 1u : 0u) | (x == a.nx ? 2u : 0u) | (y == 1 ? 4u : 0u) | (y == a.ny ? 8u : 0u) |
+         (zi == 1 ? 16u : 0u) | (zi == a.nz ? 32u : 0u);
+  // Winning face BC of link k: slot * 4 + mode, and its value.
+  auto winner = [&](int k, C &val) -> int {
+    int wn = 0;
 #pragma unroll
-    for (int k = 0; k < Q; ++k) {
-      nbm[k] = idx - offc<Q>(k, a.sx, a.sy);
-      nbp[k] = idx + offc<Q>(k, a.sx, a.sy);
+    for (int d = 0; d < 3; ++d) {
+      const int cd = Lat<Q>::c[k][d];
+      if (cd == 0) continue;
+      const int fc = 2 * d + (cd > 0 ? 0 : 1);  // the face this link enters through
+      const int cand = ((fb >> fc) & 1u) ? a.fface[fc] : 0;
+      if (cand > wn) {
+        wn = cand;
+        val = face_val<C>(a, fc, k);
+      }
+    }
+    return wn;
+  };
+  unsigned bmask = 0, cmask = 0;  // bounce-back / constant links
+  if (!PER) {
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+      C unused;
+      const int md = winner(k, unused) & 3;
+      if (md == MODE_BOUNCE) bmask |= 1u << k;
+      if (md == MODE_CONST) cmask |= 1u << k;
     }
   }
 
+  // One round of loads: the cell type and every population (bounce-back
+  // links read the other slot of their link pair, constant links nothing).
+  // All addresses lie inside the lattice for interior cells.
+  // Loads take no branch (divergent branches around loads into the same
+  // register force the compiler to wait for memory inside the branch).
+  const unsigned char code = a.code[idx];
   C v[Q];
-  unsigned skipm = 0;
-  const bool bnd = code != CODE_FLUID;
-  if (!PER && __any(bnd)) {
-    // General path: some lane of this wavefront has BC-resolved links.
-    LinkRec rec;
-    if (bnd) {
-      rec = a.rec[a.pat[idx]];
-      skipm = rec.skip;
-    } else {
 #pragma unroll
-      for (int k = 0; k < 28; ++k) rec.slot[k] = 0;
+  for (int k = 0; k < Q; ++k) {
+    const int kb = inv_dir<Q>(k);
+    const bool bb = (bmask >> k) & 1u;
+    if constexpr (WIDE) {
+      const char *pn = a.fd[ODD ? kb : k] + (ODD ? nb(k, -1) : ib);  // normal slot
+      const char *pp = a.fd[ODD ? k : kb] + (ODD ? ib : nb(k, -1));  // the other slot of the pair
+      v[k] = CD::ld(bb ? pp : pn, 0u);
+    } else {
+      // byte offsets from the first direction array
+      const unsigned on = static_cast<unsigned>(ODD ? kb : k) * a.qs32 + (ODD ? nb(k, -1) : ib);
+      const unsigned op = static_cast<unsigned>(ODD ? k : kb) * a.qs32 + (ODD ? ib : nb(k, -1));
+      v[k] = CD::ld(a.fd[0], bb ? op : on);
     }
-    const bool raw = (a.flags & STEP_F_RAW) != 0;
+  }
+  // Keep the scheduler from hoisting the wait on the cell type above the
+  // population loads.
+  __builtin_amdgcn_sched_barrier(0);
+  if (!PER && __any(fb != 0)) {
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+      if (!(((bmask | cmask) >> k) & 1u)) continue;
+      C val = C(0);
+      (void)winner(k, val);
+      v[k] = ((bmask >> k) & 1u) ? v[k] + val : val;
+    }
+  }
+
+  // Cells that are not computed run the collision on whatever they read and
+  // store nothing; cells with a link pattern re-read their links from it.
+  const bool live = code != CODE_SKIP;
+  const bool bnd = live && code != CODE_FLUID;
+  const bool general = !PER && __any(bnd);
+  unsigned skipm = 0;
+  if (general && bnd) {
+    const LinkRec rec = a.rec[a.pat[idx]];
+    skipm = rec.skip;
     const C *val = static_cast<const C *>(a.val);
-    v[0] = CD::ld(fk(0), idx);
 #pragma unroll
     for (int k = 1; k < Q; ++k) {
       const int kb = inv_dir<Q>(k);
       const int slot = raw ? 0 : rec.slot[k];
       const unsigned char md = slot ? a.mode[slot] : static_cast<unsigned char>(MODE_NONE);
       C x;
-      if (md == MODE_BOUNCE)
-        x = CD::ld(fk(ODD ? k : kb), ODD ? idx : nbm[k]) + val[slot * Q + k];
-      else if (md == MODE_CONST)
-        x = val[slot * Q + k];
-      else
-        x = CD::ld(fk(ODD ? kb : k), ODD ? nbm[k] : idx);
-      v[k] = x;
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-      const int kb = inv_dir<Q>(k);
-      v[k] = CD::ld(fk(ODD ? kb : k), ODD ? nbm[k] : idx);
+      if constexpr (WIDE) {
+        const char *pn = a.fd[ODD ? kb : k] + (ODD ? nb(k, -1) : ib);
+        const char *pp = a.fd[ODD ? k : kb] + (ODD ? ib : nb(k, -1));
+        x = CD::ld(md == MODE_BOUNCE ? pp : pn, 0u);
+      } else {
+        const unsigned on = static_cast<unsigned>(ODD ? kb : k) * a.qs32 + (ODD ? nb(k, -1) : ib);
+        const unsigned op = static_cast<unsigned>(ODD ? k : kb) * a.qs32 + (ODD ? ib : nb(k, -1));
+        x = CD::ld(a.fd[0], md == MODE_BOUNCE ? op : on);
+      }
+      const C sv = val[slot * Q + k];
+      v[k] = md == MODE_BOUNCE ? x + sv : (md == MODE_CONST ? sv : x);
     }
   }
 
-  C r, vel[3];
-  collide<Q, C>(v, C(a.w), (a.flags & STEP_F_COMPRESSIBLE) != 0, r, vel);
+  C r, m[3];
+  collide<Q, C>(v, C(a.w), (a.flags & STEP_F_COMPRESSIBLE) != 0, r, m);
 
+  if (!live) return;
+  if (general) {
 #pragma unroll
-  for (int k = 0; k < Q; ++k) {
-    if ((skipm >> k) & 1u) continue;
-    const int kb = inv_dir<Q>(k);
-    CD::st(fk(ODD ? k : kb), ODD ? nbp[k] : idx, v[k]);
+    for (int k = 0; k < Q; ++k)
+      if (!((skipm >> k) & 1u)) CD::st(a.fd[ODD ? k : inv_dir<Q>(k)], ODD ? nb(k, 1) : ib, v[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < Q; ++k) CD::st(a.fd[ODD ? k : inv_dir<Q>(k)], ODD ? nb(k, 1) : ib, v[k]);
   }
   if (a.flags & STEP_F_RHO) static_cast<M *>(a.rho)[idx] = M(r);
   if (a.flags & STEP_F_VEL) {
+    const bool comp = (a.flags & STEP_F_COMPRESSIBLE) != 0;
     M *u = static_cast<M *>(a.u);
-    u[idx] = M(vel[0]);
-    u[a.len + idx] = M(vel[1]);
-    u[2 * a.len + idx] = M(vel[2]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d * a.len + idx] = M(comp ? m[d] / r : m[d]);
   }
 }
 
@@ -267,14 +352,15 @@ __global__ __launch_bounds__(BLOCK) void k_vlinks(const VLinkArgs a) {
   if (e >= a.nlinks) return;
   const VLink l = a.links[e];
   const VLinkBC bc = a.bc[l.slot];
-  auto fk = [&](int k) -> S * { return reinterpret_cast<S *>(a.f + k * a.qs_bytes); };
+  constexpr unsigned ES = sizeof(S);
+  auto fk = [&](int k) -> char * { return a.f + k * a.qs_bytes; };
   C coef = C(0), r = C(0);
 #pragma unroll
   for (int i = 1; i < Q; ++i) {
     if (!((l.comp >> i) & 1u)) continue;
     const int kb = inv_dir<Q>(i);
     const int p = l.g + offc<Q>(i, a.sx, a.sy);
-    const C fi = a.after_odd ? CD::ld(fk(kb), l.g) : CD::ld(fk(i), p);
+    const C fi = a.after_odd ? CD::ld(fk(kb), ES * l.g) : CD::ld(fk(i), ES * p);
     coef = coef + C(Lat<Q>::w(i));
     r = r + fi * C(Lat<Q>::w(i));
   }
@@ -303,9 +389,9 @@ __global__ __launch_bounds__(BLOCK) void k_vlinks(const VLinkArgs a) {
     const int p = l.g + offc<Q>(i, a.sx, a.sy);
     const C val = r + dot_c<Q>(mom, i) * C(3.);
     if (a.after_odd)
-      CD::st(fk(i), p, val);  // next step is even: p reads slot (p, i)
+      CD::st(fk(i), ES * p, val);  // next step is even: p reads slot (p, i)
     else
-      CD::st(fk(kb), l.g, val);  // next step is odd: p reads slot (g, inv i)
+      CD::st(fk(kb), ES * l.g, val);  // next step is odd: p reads slot (g, inv i)
   }
 }
 
@@ -323,13 +409,13 @@ __global__ __launch_bounds__(BLOCK) void k_init(const InitArgs a) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) u[d] = a.u_in ? C(static_cast<const M *>(a.u_in)[d * a.len + idx]) : C(a.u0[d]);
   C uu = (C(1.5) * u[0]) * u[0];
-  uu = uu + (C(1.5) * u[1]) * u[1];
-  uu = uu + (C(1.5) * u[2]) * u[2];
+  uu = fmaT(C(1.5) * u[1], u[1], uu);
+  uu = fmaT(C(1.5) * u[2], u[2], uu);
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const C ai = i == 0 ? C(0) : dot_c<Q>(u, i);
-    const C fi = ((C(1.) + C(3.) * ai) + C(4.5) * (ai * ai)) - uu;
-    CD::st(reinterpret_cast<S *>(a.f + inv_dir<Q>(i) * a.qs_bytes), idx, r * fi);
+    const C fi = fmaT(C(4.5), ai * ai, fmaT(C(3.), ai, C(1.))) - uu;
+    CD::st(a.f + inv_dir<Q>(i) * a.qs_bytes, static_cast<unsigned>(idx) * sizeof(S), r * fi);
   }
   static_cast<M *>(a.rho)[idx] = M(r);
 #pragma unroll
@@ -351,10 +437,10 @@ __global__ __launch_bounds__(BLOCK) void k_unpermute(const UnpermuteArgs a) {
     if (a.after_odd) {
       const int src = idx + offc<Q>(i, a.sx, a.sy);
       v = (src >= 0 && src < a.len)
-              ? M(CD::ld(reinterpret_cast<const S *>(a.f + i * a.qs_bytes), src))
+              ? M(CD::ld(a.f + i * a.qs_bytes, static_cast<unsigned>(src) * sizeof(S)))
               : M(__builtin_nan(""));
     } else {
-      v = M(CD::ld(reinterpret_cast<const S *>(a.f + inv_dir<Q>(i) * a.qs_bytes), idx));
+      v = M(CD::ld(a.f + inv_dir<Q>(i) * a.qs_bytes, static_cast<unsigned>(idx) * sizeof(S)));
     }
     out[static_cast<long long>(i) * a.len + idx] = v;
   }
@@ -376,9 +462,9 @@ __global__ __launch_bounds__(BLOCK) void k_reduce(const ReduceArgs a) {
     for (int i = 0; i < Q; ++i) {
       double fi;
       if (a.after_odd)
-        fi = double(CD::ld(reinterpret_cast<const S *>(a.f + i * a.qs_bytes), idx + offc<Q>(i, a.sx, a.sy)));
+        fi = double(CD::ld(a.f + i * a.qs_bytes, static_cast<unsigned>(idx + offc<Q>(i, a.sx, a.sy)) * sizeof(S)));
       else
-        fi = double(CD::ld(reinterpret_cast<const S *>(a.f + inv_dir<Q>(i) * a.qs_bytes), idx));
+        fi = double(CD::ld(a.f + inv_dir<Q>(i) * a.qs_bytes, static_cast<unsigned>(idx) * sizeof(S)));
       fi *= Lat<Q>::w(i);
       r += fi;
 #pragma unroll
@@ -421,21 +507,20 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_final(const double *partial, i
   if (threadIdx.x < 5) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
-template <int Q, class S>
-hipError_t step_q_s(bool odd, bool per, const StepArgs &a, hipStream_t s) {
+template <int Q, class S, bool ODD, bool PER>
+hipError_t step_w(bool wide, const StepArgs &a, hipStream_t s) {
   const dim3 grid(a.nblocks), block(BLOCK);
-  if (per) {
-    if (odd)
-      hipLaunchKernelGGL((k_step<Q, S, true, true>), grid, block, 0, s, a);
-    else
-      hipLaunchKernelGGL((k_step<Q, S, false, true>), grid, block, 0, s, a);
-  } else {
-    if (odd)
-      hipLaunchKernelGGL((k_step<Q, S, true, false>), grid, block, 0, s, a);
-    else
-      hipLaunchKernelGGL((k_step<Q, S, false, false>), grid, block, 0, s, a);
-  }
+  if (wide)
+    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, false>), grid, block, 0, s, a);
   return hipGetLastError();
+}
+
+template <int Q, class S>
+hipError_t step_q_s(bool odd, bool per, bool wide, const StepArgs &a, hipStream_t s) {
+  if (per) return odd ? step_w<Q, S, true, true>(wide, a, s) : step_w<Q, S, false, true>(wide, a, s);
+  return odd ? step_w<Q, S, true, false>(wide, a, s) : step_w<Q, S, false, false>(wide, a, s);
 }
 
 template <template <int, class> class F, class... Args>
@@ -459,7 +544,9 @@ hipError_t dispatch(int q, int dtype, Args &&...args) {
 
 template <int Q, class S>
 struct StepF {
-  static hipError_t run(bool odd, bool per, const StepArgs &a, hipStream_t s) { return step_q_s<Q, S>(odd, per, a, s); }
+  static hipError_t run(bool odd, bool per, bool wide, const StepArgs &a, hipStream_t s) {
+    return step_q_s<Q, S>(odd, per, wide, a, s);
+  }
 };
 template <int Q, class S>
 struct VLinkF {
@@ -500,8 +587,8 @@ int reduce_blocks(int len) {
   return nb < RED_MAX_BLOCKS ? (nb < 1 ? 1 : nb) : RED_MAX_BLOCKS;
 }
 
-hipError_t launch_step(int q, int dtype, bool odd, bool periodic, const StepArgs &a, hipStream_t s) {
-  return dispatch<StepF>(q, dtype, odd, periodic, a, s);
+hipError_t launch_step(int q, int dtype, bool odd, bool periodic, bool wide, const StepArgs &a, hipStream_t s) {
+  return dispatch<StepF>(q, dtype, odd, periodic, wide, a, s);
 }
 hipError_t launch_vlinks(int q, int dtype, const VLinkArgs &a, hipStream_t s) {
   return dispatch<VLinkF>(q, dtype, a, s);

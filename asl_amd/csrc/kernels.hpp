@@ -17,14 +17,28 @@ enum : unsigned {
 // One fused collide+stream pass (AA pattern) over the ghost-inclusive block.
 // The launch is a 1-D grid of 256-cell chunks of the interior planes
 // x = 1..nx (rows y = 1..ny, every z including the two ghost columns).
+//
+// Links into cells of the six face layers are resolved from the cell's
+// coordinates: for each face, fface[f] = slot * 4 + mode of the last
+// executed slice BC on that face (0: none) and fval*[f][k] its value for
+// direction k.  A link k into a cell is claimed by the faces the cell lies on
+// whose inward direction e_f has c_k . e_f > 0; the highest slot wins (the
+// BC executed last).  Cells whose links this rule does not describe (next to
+// map obstacles, computed ghost cells) carry CODE_FLUID_BC / CODE_GHOST and a
+// link pattern instead.
 struct StepArgs {
-  char *f;                    // direction k array at f + k * qs_bytes
-  long long qs_bytes;         // bytes between direction arrays
+  char *fd[27];               // base of each direction array
+  unsigned qs32;              // bytes between direction arrays (if all fit in 4 GiB)
   int sx, sy;                 // strides of the layout: Sy*Sz and Sz
   int nx, ny, nz;             // interior size
   int len;                    // ghost-inclusive cell count
   int bpp;                    // 256-cell chunks per plane
   int nblocks;                // bpp * nx
+  unsigned long long zmagic;  // ceil(2^40 / sy): row = (off * zmagic) >> 40
+  int remap;                  // 1: XCD-contiguous chunk order (see xcd_remap)
+  int fface[6];               // per face: slot * 4 + mode of its winning slice BC
+  float fval32[6][27];        // per face and direction: addend / constant (FP32 lattices)
+  double fval64[6][27];       // the same for FP64 lattices
   const unsigned char *code;  // packed cell types
   const unsigned short *pat;  // link pattern of CODE_FLUID_BC / CODE_GHOST cells
   const LinkRec *rec;         // link patterns
@@ -86,7 +100,8 @@ struct ReduceArgs {
   double *partial;  // 5 per block
 };
 
-hipError_t launch_step(int q, int dtype, bool odd, bool periodic, const StepArgs &a, hipStream_t s);
+// wide: the populations span more than 4 GiB (64-bit per-lane addresses).
+hipError_t launch_step(int q, int dtype, bool odd, bool periodic, bool wide, const StepArgs &a, hipStream_t s);
 hipError_t launch_vlinks(int q, int dtype, const VLinkArgs &a, hipStream_t s);
 hipError_t launch_init(int q, int dtype, const InitArgs &a, hipStream_t s);
 hipError_t launch_unpermute(int q, int dtype, const UnpermuteArgs &a, hipStream_t s);

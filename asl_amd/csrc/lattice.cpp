@@ -134,6 +134,12 @@ struct aslhip_lattice {
   bool fresh = false;        // no step since init
   bool bc_since_init = false;
   bool next_odd = true;
+  // launch tuning
+  int tune_remap = 1;
+  // face rule: per face slot * 4 + mode of the winning slice BC, and values
+  int fface[6] = {0, 0, 0, 0, 0, 0};
+  float fval32[6][27] = {};
+  double fval64[6][27] = {};
   // stats
   long long n_counted = 0, n_computed = 0, n_boundary = 0, n_patterns = 0;
   std::vector<unsigned char> h_code;
@@ -321,6 +327,45 @@ int finalize_impl(aslhip_lattice *h) {
           }
     }
 
+    // Face rule: the last executed slice BC on each face claims the links
+    // that enter through that face (see StepArgs).
+    int fslot[6] = {0, 0, 0, 0, 0, 0};
+    for (int o = 0; o < nord; ++o)
+      if (!ob[o]->is_map())
+        for (int fc = 0; fc < 6; ++fc)
+          if (ob[o]->faces & (1u << fc)) fslot[fc] = o + 1;
+    auto face_rule = [&](int x, int y, int z, int k) -> int {
+      const int p[3] = {x, y, z};
+      int best = 0;
+      for (int d = 0; d < 3; ++d) {
+        const int cd = t.c[k][d];
+        if (cd == 0) continue;
+        const int fc = 2 * d + (cd > 0 ? 0 : 1);
+        const bool on = (fc % 2 == 0) ? p[d] == 1 : p[d] == h->n[d];
+        if (on && fslot[fc] > best) best = fslot[fc];
+      }
+      return best;
+    };
+    for (int fc = 0; fc < 6; ++fc) {
+      h->fface[fc] = 0;
+      for (int k = 0; k < 27; ++k) {
+        h->fval32[fc][k] = 0.f;
+        h->fval64[fc][k] = 0.;
+      }
+      if (!fslot[fc]) continue;
+      const BCDesc &b = *ob[fslot[fc] - 1];
+      unsigned char md = MODE_NONE;
+      double v64[27];
+      float v32[27];
+      bc_values<double>(h, b, md, v64);
+      bc_values<float>(h, b, md, v32);
+      h->fface[fc] = fslot[fc] * 4 + md;
+      for (int k = 0; k < Q; ++k) {
+        h->fval32[fc][k] = v32[k];
+        h->fval64[fc][k] = v64[k];
+      }
+    }
+
     // Cells that some map BC treats as ghost nodes (map <= 0), any layer.
     std::vector<unsigned char> solid_any(len, 0);
     for (int o : map_pos)
@@ -342,7 +387,9 @@ int finalize_impl(aslhip_lattice *h) {
           for (int k = 1; k < Q && near; ++k) {
             const int s = resolve(x, y, z, k);
             r.slot[k] = static_cast<unsigned char>(s);
-            any = any || s != 0;
+            // the step kernel resolves this link itself unless the face
+            // rule disagrees with the full resolution
+            any = any || s != face_rule(x, y, z, k);
           }
           // Outgoing link k of a computed ghost cell: skip it if its receiver
           // resolves it (computation nodes are never the source of a BC entry).
@@ -491,8 +538,16 @@ int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEven
     }
   }
   StepArgs a;
-  a.f = h->f;
-  a.qs_bytes = h->qs_bytes;
+  for (int k = 0; k < 27; ++k) a.fd[k] = k < h->Q ? h->f + k * h->qs_bytes : nullptr;
+  const bool wide = h->qs_bytes * h->Q > 0xffffffffLL;
+  a.qs32 = wide ? 0u : static_cast<unsigned>(h->qs_bytes);
+  for (int fc = 0; fc < 6; ++fc) {
+    a.fface[fc] = h->fface[fc];
+    for (int k = 0; k < 27; ++k) {
+      a.fval32[fc][k] = h->fval32[fc][k];
+      a.fval64[fc][k] = h->fval64[fc][k];
+    }
+  }
   a.sx = h->sx;
   a.sy = h->sy;
   a.nx = h->n[0];
@@ -501,6 +556,8 @@ int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEven
   a.len = h->len;
   a.bpp = (h->n[1] * h->sy + 255) / 256;
   a.nblocks = a.bpp * h->n[0];
+  a.zmagic = ((1ULL << 40) + (unsigned long long)h->sy - 1) / (unsigned long long)h->sy;
+  a.remap = h->tune_remap;
   a.code = h->code;
   a.pat = h->pat;
   a.rec = h->rec;
@@ -518,7 +575,7 @@ int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEven
     }
     a.flags = fl;
     if (ev) HIPCHK(hipEventRecord((*ev)[2 * s], h->stream));
-    HIPCHK(launch_step(h->Q, h->dtype, h->next_odd, h->periodic, a, h->stream));
+    HIPCHK(launch_step(h->Q, h->dtype, h->next_odd, h->periodic, wide, a, h->stream));
     if (ev) HIPCHK(hipEventRecord((*ev)[2 * s + 1], h->stream));
     const bool was_odd = h->next_odd;
     h->next_odd = !h->next_odd;
@@ -589,6 +646,12 @@ int aslhip_lattice_create(aslhip_lattice **out, int nx, int ny, int nz, int q, i
   h->melem = h->celem = dtype == ASLHIP_F64 ? 8 : 4;
   // direction arrays 256-byte aligned
   const long long qs_elems = (len + 127) / 128 * 128;
+  // The step kernel finds a cell's row as (off * ceil(2^40/Sz)) >> 40, exact
+  // while (cells per plane) * Sz < 2^40.
+  if ((double)S1 * S2 * S2 >= 1099511627776.0)
+    return fail(ASLHIP_EINVAL, "ASL ERROR: block planes too large");
+  if (qs_elems * (dtype == ASLHIP_F64 ? 8 : 4) > 0xffffffffLL)
+    return fail(ASLHIP_EINVAL, "ASL ERROR: block too large: one direction array must stay below 4 GiB");
   h->qs_bytes = qs_elems * (long long)h->selem;
   hipError_t e = hipStreamCreate(&h->stream);
   if (e == hipSuccess) e = hipMalloc(&h->f, (size_t)h->qs_bytes * q);
@@ -835,6 +898,15 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]) {
   out[2] = h->n_boundary;
   out[3] = h->n_patterns;
   return 0;
+}
+
+int aslhip_set_tuning(aslhip_lattice *h, int knob, int value) {
+  if (int r = check(h)) return r;
+  if (knob == ASLHIP_TUNE_XCD_REMAP) {
+    h->tune_remap = value ? 1 : 0;
+    return 0;
+  }
+  return fail(ASLHIP_EINVAL, "ASL ERROR: unknown tuning knob %d", knob);
 }
 
 int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags, double *ms_total, double *ms_kernel) {

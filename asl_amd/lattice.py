@@ -114,6 +114,9 @@ class Lattice:
     def synchronize(self):
         call("aslhip_synchronize", self.h)
 
+    def set_tuning(self, knob, value):
+        call("aslhip_set_tuning", self.h, int(knob), int(value))
+
     def time_steps(self, nsteps, write_macro=False):
         tot = ctypes.c_double(0)
         ker = ctypes.c_double(0)

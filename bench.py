@@ -1,0 +1,168 @@
+#!/usr/bin/env python3
+"""Throughput of the LBGK hot path on one MI355X: D3Q19 MLUPS and HBM roofline.
+
+Workload (the configuration the headline metric is quoted on): D3Q19 LBGK
+channel flow past a sphere, 256x128x128 cells, FP32, tau = 0.56 (nu = 0.02),
+sphere of radius 16 at (64, 64, 64) from a normalised level-set map
+(no-slip), velocity inlet u = 0.04 on X0, pressure outlet rho = 1 on XE,
+no-slip side walls; initial u = (0.04, 0, 0) + 1e-3 * U[-1, 1] per cell and
+component from a seeded generator, u = 0 inside the sphere.  A step is one
+fused collide+stream pass over the whole lattice with every BC applied.
+
+The timed region holds exactly --steps steps with the populations already in
+HBM; it is bracketed by stream synchronisations.  ``value`` is MLUPS over
+computation nodes (solid and ghost cells are not counted).  The roofline
+entry is the step kernel's algorithmic HBM bytes per launch over its average
+launch duration, measured with HIP events recorded on the lattice's stream.
+``cpu_baseline`` times the CPU oracle (the C++ restatement of ASL's LBGK) on
+a bounded sample of the same lattice with the host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "D3Q19 MLUPS & % HBM roofline; coarse+closure vs plain fine at equal KE error"
+HBM_PEAK_GBS = 8000.0
+SIZE = (256, 128, 128)
+NU = (0.56 - 0.5) / 3.0
+U_IN = 0.04
+SEED = 20261202
+
+
+def setup_fields():
+    """Sphere map and the perturbed initial velocity (float32, ASL layout)."""
+    from asl_amd import geometry
+    mapv = geometry.normalize(geometry.sphere(SIZE, 16.0, (64.0, 64.0, 64.0)))
+    S = tuple(s + 2 for s in SIZE)
+    rng = np.random.default_rng(SEED)
+    u = np.zeros((3,) + S, dtype=np.float32)
+    u[0] = U_IN
+    u += np.float32(1e-3) * rng.uniform(-1.0, 1.0, size=u.shape).astype(np.float32)
+    u[:, mapv <= 0] = 0.0
+    return mapv, u
+
+
+def build_gpu(mapv, u):
+    import asl_amd
+    from asl_amd import _lib
+    L = asl_amd.Lattice(SIZE, 19, dtype="f32", nu=NU)
+    L.init_equilibrium(1.0, u)
+    bcs = [L.add_slice_bc(_lib.BC_CONSTANT_VELOCITY, _lib.X0, u=(U_IN, 0.0, 0.0)),
+           L.add_slice_bc(_lib.BC_CONSTANT_PRESSURE, _lib.XE, p=1.0),
+           L.add_slice_bc(_lib.BC_NOSLIP, _lib.Y0 | _lib.YE | _lib.Z0 | _lib.ZE),
+           L.add_map_bc(_lib.BC_NOSLIP_MAP, mapv)]
+    for b in bcs:
+        L.execute_bc(b)
+    L.finalize()
+    return L
+
+
+def cpu_baseline(mapv, u, budget_s=15.0):
+    """The CPU oracle (FP32, ASL's algorithm) on the same lattice for ~budget_s seconds."""
+    import oracle
+    from oracle.lbgk import set_threads
+    threads = set_threads(min(16, os.cpu_count() or 1))
+    O = oracle.OracleLattice(SIZE, 19, fp64=False, nu=NU)
+    O.init_f(1.0, u.astype(np.float64))
+    bcs = [O.add_slice_bc(O.CONSTANT_VELOCITY, 1, u=(U_IN, 0.0, 0.0)),
+           O.add_slice_bc(O.CONSTANT_PRESSURE, 2, p=1.0),
+           O.add_slice_bc(O.NOSLIP, 4 | 8 | 16 | 32),
+           O.add_map_bc(O.NOSLIP_MAP, mapv)]
+    for b in bcs:
+        O.execute_bc(b)
+    O.run(1, bcs)  # warm-up
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        O.run(1, bcs)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 2000:
+            break
+    counted = int(((mapv > 0)[1:-1, 1:-1, 1:-1]).sum())
+    return {"value": round(counted * steps / el / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": "CPU oracle (C++/OpenMP, FP32, ASL's pool-shift LBGK + BCs) on the same 256x128x128 "
+                      "sphere lattice, %d steps in %.1f s" % (steps, el)}
+
+
+def profile_traffic():
+    """HBM bytes per launch of the step kernel from the committed rocprofv3 summary, or None."""
+    path = os.path.join(ROOT, "profiles", "lbgk_step_d3q19_f32_sphere_256x128x128_traffic.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline sample")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this LBGK build runs on one GPU only; --gpus %d is not supported "
+              "(the multi-GPU slab decomposition is a design, not built)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        sys.exit("bench.py: --steps must be >= 1 and --warmup >= 0")
+
+    mapv, u = setup_fields()
+    L = build_gpu(mapv, u)
+    st = L.stats()
+    counted = st["counted"]
+    interior = int(np.prod(SIZE))
+    if args.warmup:
+        L.step(args.warmup, write_macro=False)
+    L.synchronize()
+    t0 = time.perf_counter()
+    ms_total, ms_kernel = L.time_steps(args.steps, write_macro=False)
+    L.synchronize()
+    wall = time.perf_counter() - t0
+    mlups = counted * args.steps / wall / 1e6
+
+    # algorithmic bytes per launch: 19 FP32 reads + 19 FP32 writes per
+    # computation node, plus the one-byte cell type of every interior cell
+    bytes_per_launch = 152.0 * counted + 1.0 * interior
+    kern_s = ms_kernel / args.steps / 1e3
+    achieved = bytes_per_launch / kern_s / 1e9
+    traffic = profile_traffic()
+    line = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "D3Q19 LBGK channel flow past a sphere, 256x128x128, FP32, tau=0.56, "
+                               "sphere r=16 at (64,64,64), inlet u=0.04 (X0), outlet rho=1 (XE), "
+                               "no-slip walls", "lattice": "D3Q19", "cells": interior,
+                   "computation_nodes": counted, "storage": "FP32 SoA, in-place AA pattern"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel_ms_avg": round(kern_s * 1e3, 5), "bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(mapv, u)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

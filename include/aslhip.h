@@ -180,6 +180,13 @@ int aslhip_reduce_conservation(aslhip_lattice *h, double out[5]);
  * cells, distinct link patterns. */
 int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
 
+/* Launch tuning of the step kernel (no effect on results).  The autotuner
+ * in asl_amd/autotune.py measures the knobs and keeps the fastest setting.
+ *   ASLHIP_TUNE_XCD_REMAP: 1 = each XCD takes one contiguous range of the
+ *   lattice (default), 0 = hardware round-robin order. */
+#define ASLHIP_TUNE_XCD_REMAP 1
+int aslhip_set_tuning(aslhip_lattice *h, int knob, int value);
+
 /* Times nsteps steps with HIP events recorded on the lattice's stream.
  * ms_total: wall time of the whole call between the events; ms_kernel:
  * summed time of the step kernels (events around each launch). */

@@ -11,7 +11,9 @@
 // (src/acl/aclMath/aclVectorOfElementsOperations.cxx:336-364: a dot product is
 // a left-to-right sum a0*b0 + a1*b1 + ...).  Terms whose constant factor is an
 // exact zero (c_i components) are left out: for finite populations x + (+-0)
-// == x, so the value is unchanged.
+// == x, so the value is unchanged.  Sums of products x*y + z are contracted
+// to one fma (OpenCL's default FP_CONTRACT ON), at the same sites as in the
+// GPU kernel; built with -mfma so std::fma is a single instruction.
 #include "lbgk_oracle.h"
 
 #include <cmath>
@@ -198,6 +200,9 @@ struct Lattice {
   }
 
   // The collision of one cell, LBGK::init0 (src/num/aslLBGK.cxx:187-235).
+  // Every x*y + z of ASL's expression tree is contracted to fma(x, y, z), as an
+  // OpenCL compiler does under the default FP_CONTRACT ON; the GPU kernel
+  // contracts the same sites, so both round identically.
   inline void collide(T *f, T w, T &rho_o, T *v_o) const {
     const int Q = vt.Q;
     // momentum = computeMomentum(fOld) (:51-63), pRho = computeRho(fOld) (:46-49)
@@ -207,14 +212,13 @@ struct Lattice {
       T acc = 0;
       for (int j = 0; j < Q; ++j) {
         if (vt.c[j][d] == 0) continue;
-        T t = f[j] * cwc[j][d];
-        acc = first ? t : acc + t;
+        acc = first ? f[j] * cwc[j][d] : std::fma(f[j], cwc[j][d], acc);
         first = false;
       }
       m[d] = acc;
     }
     T r = f[0] * cw[0];
-    for (int j = 1; j < Q; ++j) r = r + f[j] * cw[j];
+    for (int j = 1; j < Q; ++j) r = std::fma(f[j], cw[j], r);
     rho_o = r;
     if (compressible)
       for (int d = 0; d < 3; ++d) v_o[d] = m[d] / r;
@@ -222,8 +226,8 @@ struct Lattice {
       for (int d = 0; d < 3; ++d) v_o[d] = m[d];
     // 1.5 * momentum * momentum (dot product), then / pRho (compressible)
     T mm = (T(1.5) * m[0]) * m[0];
-    mm = mm + (T(1.5) * m[1]) * m[1];
-    mm = mm + (T(1.5) * m[2]) * m[2];
+    mm = std::fma(T(1.5) * m[1], m[1], mm);
+    mm = std::fma(T(1.5) * m[2], m[2], mm);
     const T one_w = T(1.) - w;
     const T mmr = compressible ? mm / r : mm;
     for (int i = 0; i < Q; ++i) {
@@ -240,11 +244,11 @@ struct Lattice {
       if (i == 0) {
         feq = r - mmr;  // rho + 3*0 + 4.5*0/rho - mm/rho
       } else if (compressible) {
-        feq = ((r + T(3.) * a) + (T(4.5) * (a * a)) / r) - mmr;
+        feq = (std::fma(T(3.), a, r) + (T(4.5) * (a * a)) / r) - mmr;
       } else {
-        feq = ((r + T(3.) * a) + T(4.5) * (a * a)) - mmr;
+        feq = std::fma(T(4.5), a * a, std::fma(T(3.), a, r)) - mmr;
       }
-      f[i] = f[i] * one_w + w * feq;  // fOld*(1-w) + w*fEq
+      f[i] = std::fma(f[i], one_w, w * feq);  // fOld*(1-w) + w*fEq
     }
   }
 
@@ -299,11 +303,11 @@ struct Lattice {
   // f_i = rho * (1 + 3 a_i + 4.5 a_i^2 - 1.5 u.u), a_i = u . c_i
   void init_cell(long j, T r, const T *u) {
     T uu = (T(1.5) * u[0]) * u[0];
-    uu = uu + (T(1.5) * u[1]) * u[1];
-    uu = uu + (T(1.5) * u[2]) * u[2];
+    uu = std::fma(T(1.5) * u[1], u[1], uu);
+    uu = std::fma(T(1.5) * u[2], u[2], uu);
     for (int i = 0; i < vt.Q; ++i) {
-      T a = dot_c(u, i);
-      T fi = ((T(1.) + T(3.) * a) + T(4.5) * (a * a)) - uu;
+      T a = i == 0 ? T(0) : dot_c(u, i);
+      T fi = std::fma(T(4.5), a * a, std::fma(T(3.), a, T(1.))) - uu;
       win(i)[j] = r * fi;
     }
     rho[j] = r;

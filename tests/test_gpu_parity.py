@@ -1,0 +1,170 @@
+"""GPU parity: the HIP path (through the C ABI) vs the CPU oracle.
+
+Both evaluate the same operations in the same order with the same
+contractions, so results are compared bit for bit on computation nodes;
+the contract of the port is ρ and u within 1e-5 (asserted as well).
+FP32 results are also compared with FP64 runs (oracle and GPU) to state
+the rounding error of the method itself.
+"""
+import numpy as np
+import pytest
+
+import asl_amd
+from asl_amd import _lib, geometry
+from gpu_common import add_bc, compare, pair, run
+
+pytestmark = pytest.mark.gpu
+ALL_BUT_ZE = _lib.X0 | _lib.XE | _lib.Y0 | _lib.YE | _lib.Z0
+
+
+def cavity(n, steps, q=19, dtype="f32", lid_first=True, pre_execute=True, nu=(0.6 - 0.5) / 3):
+    G, O = pair((n, n, n), q, nu=nu, dtype=dtype)
+    G.init_equilibrium(1.0, (0, 0, 0))
+    O.init_f(1.0, (0, 0, 0))
+    lid = add_bc(G, O, _lib.BC_CONSTANT_VELOCITY, _lib.ZE, u=(0.05, 0, 0))
+    walls = add_bc(G, O, _lib.BC_NOSLIP, ALL_BUT_ZE)
+    run(G, O, steps, [lid, walls] if lid_first else [walls, lid], pre_execute)
+    return G, O
+
+
+def test_cavity_64_1000_steps_fp32_bitwise():
+    """The 64^3 lid-driven cavity (tau = 0.6, lid 0.05 on ZE), 1000 FP32 steps."""
+    G, O = cavity(64, 1000)
+    d = compare(G, O)
+    print("cavity 64^3 x 1000 FP32 GPU vs oracle:", d)
+    assert d["cells"] == 64 ** 3
+    assert d["rho"] == 0.0 and d["u"] == 0.0 and d["f"] == 0.0
+    assert d["rho"] <= 1e-5 and d["u"] <= 1e-5 * d["umax"]
+
+
+def test_cavity_fp32_error_vs_fp64():
+    """The FP32 rounding error of the method, against FP64 runs (oracle and GPU)."""
+    G32, O32 = cavity(32, 400)
+    G64, O64 = cavity(32, 400, dtype="f64")
+    d64 = compare(G64, O64)
+    assert d64["rho"] == 0.0 and d64["u"] == 0.0  # FP64 GPU == FP64 oracle
+    m = np.zeros(G32.S, dtype=bool)
+    m[1:-1, 1:-1, 1:-1] = True
+    r32, r64 = G32.rho()[m].astype(float), G64.rho()[m]
+    u32, u64 = G32.velocity()[:, m].astype(float), G64.velocity()[:, m]
+    er = np.abs(r32 - r64).max()
+    eu = np.abs(u32 - u64).max() / np.abs(u64).max()
+    print("FP32 vs FP64 after 400 steps: max|drho| %.2e, max|du|/max|u| %.2e" % (er, eu))
+    assert er < 1e-5 and eu < 1e-4
+
+
+@pytest.mark.parametrize("q", [15, 19, 27])
+@pytest.mark.parametrize("lid_first", [True, False])
+def test_cavity_templates_and_order(q, lid_first):
+    G, O = cavity(12, 60, q=q, lid_first=lid_first)
+    d = compare(G, O)
+    assert d["rho"] == 0.0 and d["u"] == 0.0 and d["f"] == 0.0, d
+
+
+def test_first_step_reads_raw_ghosts_when_no_bc_was_executed():
+    # ASL pulls the initial ghost values in the first step if no BC ran yet
+    G, O = cavity(10, 25, pre_execute=False)
+    d = compare(G, O)
+    assert d["rho"] == 0.0 and d["u"] == 0.0, d
+
+
+def test_pure_streaming_bitwise():
+    # nu = inf: w = 0, populations only stream and bounce
+    n = 12
+    S = (n + 2,) * 3
+    rng = np.random.default_rng(2)
+    u = (0.05 * rng.standard_normal((3,) + S)).astype(np.float32)
+    rho = (1 + 0.05 * rng.standard_normal(S)).astype(np.float32)
+    G, O = pair((n, n, n), 19, nu=float("inf"))
+    G.init_equilibrium(rho, u)
+    O.init_f(rho.astype(float), u.astype(float))
+    walls = add_bc(G, O, _lib.BC_NOSLIP, 63)
+    run(G, O, 37, [walls])
+    d = compare(G, O)
+    assert d["f"] == 0.0 and d["rho"] == 0.0, d
+
+
+@pytest.mark.parametrize("q", [15, 19, 27])
+@pytest.mark.parametrize("compressible", [True, False])
+def test_all_bc_kinds(q, compressible):
+    size = (24, 14, 13)
+    sphere = geometry.normalize(geometry.sphere(size, 3.2, (8.0, 6.5, 6.0)))
+    pmap = geometry.normalize(geometry.sphere(size, 2.0, (17.0, 7.0, 6.0)))
+    G, O = pair(size, q, nu=0.04, compressible=compressible)
+    G.init_equilibrium(1.0, (0.03, 0, 0))
+    O.init_f(1.0, (0.03, 0, 0))
+    bcs = [add_bc(G, O, _lib.BC_CONSTANT_VELOCITY, _lib.X0, u=(0.03, 0, 0)),
+           add_bc(G, O, _lib.BC_CONSTANT_PRESSURE, _lib.XE, p=1.0),
+           add_bc(G, O, _lib.BC_NOSLIP, _lib.Y0 | _lib.YE),
+           add_bc(G, O, _lib.BC_CONSTANT_PRESSURE_VELOCITY, _lib.Z0 | _lib.ZE, p=1.001, u=(0.02, 0.0, 0.0)),
+           add_bc(G, O, _lib.BC_NOSLIP_MAP, mapv=sphere),
+           add_bc(G, O, _lib.BC_PRESSURE_VELOCITY_MAP, mapv=pmap, p=1.002, u=(0.0, 0.01, 0.0))]
+    run(G, O, 80, bcs)
+    d = compare(G, O)
+    assert d["rho"] == 0.0 and d["u"] == 0.0 and d["f"] == 0.0, d
+
+
+@pytest.mark.parametrize("rotating", [False, True])
+def test_velocity_map(rotating):
+    # BCVelocityMap: moving obstacle surface (constant or rotating wall)
+    size = (20, 16, 14)
+    body = geometry.normalize(geometry.sphere(size, 3.5, (9.0, 7.5, 6.5)))
+    G, O = pair(size, 19, nu=0.05)
+    G.init_equilibrium(1.0, (0, 0, 0))
+    O.init_f(1.0, (0, 0, 0))
+    rot = (0.0, 0.0, 0.01, 9.0, 7.5, 6.5) if rotating else None
+    bcs = [add_bc(G, O, _lib.BC_NOSLIP, 63),
+           add_bc(G, O, _lib.BC_VELOCITY_MAP, mapv=body, u=(0.02, 0.0, 0.01), rot=rot)]
+    run(G, O, 60, bcs)
+    d = compare(G, O)
+    assert d["rho"] == 0.0 and d["u"] == 0.0 and d["f"] == 0.0, d
+    assert d["umax"] > 1e-3
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_periodic_taylor_green(q):
+    N, nu, u0 = 24, 0.02, 0.03
+    size = (N, N, N)
+    S = tuple(s + 2 for s in size)
+    k = 2 * np.pi / N
+    x = (np.arange(S[0]) - 1.0)[:, None, None]
+    y = (np.arange(S[1]) - 1.0)[None, :, None]
+    z = (np.arange(S[2]) - 1.0)[None, None, :]
+    u = np.zeros((3,) + S, dtype=np.float32)
+    u[0] = u0 * np.sin(k * x) * np.cos(k * y) * np.cos(k * z)
+    u[1] = -u0 * np.cos(k * x) * np.sin(k * y) * np.cos(k * z)
+    G, O = pair(size, q, nu=nu, periodic=True)
+    G.init_equilibrium(1.0, u)
+    O.init_f(1.0, u.astype(float))
+    G.step(100)
+    O.run(100)
+    d = compare(G, O)
+    assert d["rho"] == 0.0 and d["u"] == 0.0 and d["f"] == 0.0, d
+
+
+def test_step_chunks_equal_single_steps():
+    G1, _ = cavity(10, 0)
+    G2, _ = cavity(10, 0)
+    G1.step(17)
+    for _ in range(17):
+        G2.step(1)
+    assert np.array_equal(G1.f(), G2.f())
+    assert np.array_equal(G1.rho(), G2.rho())
+
+
+def test_conservation_reduction_matches_oracle():
+    G, O = cavity(20, 150, dtype="f64")
+    ct = G.cell_types()
+    mask = ((ct == _lib.CODE_FLUID) | (ct == _lib.CODE_FLUID_BC)).astype(np.uint8)
+    a = G.conservation()
+    b = O.conservation(mask)
+    assert np.allclose(a, b, rtol=1e-12, atol=1e-15), (a, b)
+
+
+def test_undefined_ghost_link_is_an_error():
+    G = asl_amd.Lattice((8, 8, 8), 19, nu=0.1)
+    G.init_equilibrium(1.0, (0, 0, 0))
+    b = G.add_slice_bc(_lib.BC_NOSLIP, _lib.X0 | _lib.XE | _lib.Y0 | _lib.YE | _lib.Z0)  # no ZE
+    G.execute_bc(b)
+    with pytest.raises(asl_amd.ASLError, match="ASL ERROR"):
+        G.step(1)
