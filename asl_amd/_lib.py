@@ -15,6 +15,7 @@ X0, XE, Y0, YE, Z0, ZE = 1, 2, 4, 8, 16, 32
 BC_NOSLIP, BC_CONSTANT_PRESSURE, BC_CONSTANT_VELOCITY, BC_CONSTANT_PRESSURE_VELOCITY = 1, 2, 3, 4
 BC_NOSLIP_MAP, BC_PRESSURE_VELOCITY_MAP, BC_VELOCITY_MAP = 5, 6, 7
 STEP_WRITE_MACRO = 1
+STEP_DEFER = 2
 CODE_FLUID, CODE_FLUID_BC, CODE_SKIP, CODE_GHOST = 0, 1, 2, 3
 TUNE_XCD_REMAP = 1
 

@@ -1,0 +1,121 @@
+// aslLBGKBC.h — LBGK boundary conditions (src/num/aslLBGKBC.h:45-336).
+//
+// init() registers the BC with the lattice (include/aslhip.h); execute()
+// records the driver's execute order, which decides which BC fills a ghost
+// link that several BCs claim (the last one executed).  The fused step
+// applies every BC itself.
+#ifndef ASLLBGKBC_H
+#define ASLLBGKBC_H
+
+#include <memory>
+#include <vector>
+
+#include "../acl/aclGenerators.h"
+#include "../data/aslDataWithGhostNodes.h"
+#include "../math/aslPositionFunction.h"
+#include "aslBCond.h"
+#include "aslLBGK.h"
+
+namespace asl {
+
+class BCLBGKCommon : public BCond {
+ protected:
+  SPLBGK num;
+  int kind_;
+  double p_ = 0., u_[3] = {0., 0., 0.};
+  int id_ = -1;
+
+ public:
+  BCLBGKCommon(SPLBGK nm, int kind);
+  virtual void init();
+  virtual void execute();
+};
+
+class BCNoSlip : public BCLBGKCommon {
+ public:
+  explicit BCNoSlip(SPLBGK nm);
+};
+
+class BCConstantPressure : public BCLBGKCommon {
+ public:
+  BCConstantPressure(SPLBGK nm, const acl::VectorOfElements &p);
+};
+
+class BCConstantVelocity : public BCLBGKCommon {
+ public:
+  BCConstantVelocity(SPLBGK nm, const acl::VectorOfElements &v);
+};
+
+class BCConstantPressureVelocity : public BCLBGKCommon {
+ public:
+  BCConstantPressureVelocity(SPLBGK nm, const acl::VectorOfElements &p, const acl::VectorOfElements &v);
+};
+
+class BCondWithMap : public NumMethod {
+ protected:
+  SPLBGK num;
+  SPAbstractDataWithGhostNodes map, computationalDomain;
+  int kind_;
+  double p_ = 0., u_[3] = {0., 0., 0.};
+  bool has_rot_ = false;
+  double rot_[6] = {0., 0., 0., 0., 0., 0.};
+  int id_ = -1;
+
+ public:
+  BCondWithMap(SPLBGK nm, int kind, SPAbstractDataWithGhostNodes m, SPAbstractDataWithGhostNodes cd = nullptr);
+  virtual void init();
+  virtual void execute();
+};
+
+class BCNoSlipMap : public BCondWithMap {
+ public:
+  BCNoSlipMap(SPLBGK nm, SPAbstractDataWithGhostNodes map);
+};
+
+class BCVelocityMap : public BCondWithMap {
+ public:
+  BCVelocityMap(SPLBGK nm, SPPositionFunction v, SPAbstractDataWithGhostNodes map);
+  BCVelocityMap(SPLBGK nm, SPPositionFunction v, SPAbstractDataWithGhostNodes map,
+                SPAbstractDataWithGhostNodes computationalDomain);
+};
+
+class BCConstantPressureVelocityMap : public BCondWithMap {
+ public:
+  BCConstantPressureVelocityMap(SPLBGK nm, acl::VectorOfElements p, SPAbstractDataWithGhostNodes map);
+  BCConstantPressureVelocityMap(SPLBGK nm, acl::VectorOfElements p, acl::VectorOfElements v,
+                                SPAbstractDataWithGhostNodes map);
+};
+
+/// Visualisation fill of a macroscopic field at map ghost nodes, on the host
+/// view (ASL's BCConstantValueMiddlePointMap, src/num/aslBasicBC.cxx:171-230):
+/// v(g) = sum_i lc_i (2 value - v(g + c_i)) / max(sum_i lc_i, 0.01) over the
+/// computation-node neighbours; with gradient = true the zero-gradient fill
+/// v(g) = sum_i lc_i v(g + c_i) / sum_i lc_i.
+class BCMacroFillMap : public NumMethod {
+ public:
+  BCMacroFillMap(SPLBGK nm, SPAbstractDataWithGhostNodes data, SPAbstractDataWithGhostNodes map, bool gradient);
+  virtual void init() {}
+  virtual void execute();
+
+ private:
+  SPLBGK num;
+  SPAbstractDataWithGhostNodes data, map;
+  bool gradient;
+};
+
+SPBCond generateBCNoSlip(SPLBGK nm, const std::vector<SlicesNames> &sl);
+SPBCond generateBCConstantVelocity(SPLBGK nm, AVec<> v, const std::vector<SlicesNames> &sl);
+SPBCond generateBCConstantPressure(SPLBGK nm, double p, const std::vector<SlicesNames> &sl);
+SPBCond generateBCConstantPressureVelocity(SPLBGK nm, double p, AVec<> v, const std::vector<SlicesNames> &sl);
+SPNumMethod generateBCConstantPressure(SPLBGK nm, double p, SPAbstractDataWithGhostNodes map);
+SPNumMethod generateBCConstantPressureVelocity(SPLBGK nm, double p, AVec<> v, SPAbstractDataWithGhostNodes map);
+SPNumMethod generateBCNoSlip(SPLBGK nm, SPAbstractDataWithGhostNodes map);
+SPNumMethod generateBCNoSlipVel(SPLBGK nmU, SPAbstractDataWithGhostNodes map);
+SPNumMethod generateBCNoSlipRho(SPLBGK nmU, SPAbstractDataWithGhostNodes map);
+SPNumMethod generateBCVelocity(SPLBGK nm, SPPositionFunction v, SPAbstractDataWithGhostNodes map);
+SPNumMethod generateBCVelocity(SPLBGK nm, SPPositionFunction v, SPAbstractDataWithGhostNodes map,
+                               SPAbstractDataWithGhostNodes computationalDomain);
+
+}  // namespace asl
+
+#endif

@@ -191,7 +191,7 @@ __device__ __forceinline__ double face_val<double>(const StepArgs &a, int f, int
 }
 
 template <int Q, class S, bool ODD, bool PER, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_step(const StepArgs a) {
+__global__ __launch_bounds__(STEP_BLOCK) void k_step(const StepArgs a) {
   using CD = Codec<S>;
   using C = typename CD::C;
   using M = typename CD::M;
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(const StepArgs a) {
   const int L = a.remap ? xcd_remap(blockIdx.x, a.nblocks) : static_cast<int>(blockIdx.x);
   const int plane = L / a.bpp;
   const int chunk = L - plane * a.bpp;
-  const unsigned off = static_cast<unsigned>(chunk * BLOCK) + threadIdx.x;
+  const unsigned off = static_cast<unsigned>(chunk * STEP_BLOCK) + threadIdx.x;
   if (off >= static_cast<unsigned>(a.ny * a.sy)) return;
   // Row and column inside the plane (exact: off * sy < 2^40, see StepArgs::zmagic).
   const unsigned row = static_cast<unsigned>((static_cast<unsigned long long>(off) * a.zmagic) >> 40);
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_final(const double *partial, i
 
 template <int Q, class S, bool ODD, bool PER>
 hipError_t step_w(bool wide, const StepArgs &a, hipStream_t s) {
-  const dim3 grid(a.nblocks), block(BLOCK);
+  const dim3 grid(a.nblocks), block(STEP_BLOCK);
   if (wide)
     hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true>), grid, block, 0, s, a);
   else

@@ -6,6 +6,9 @@
 
 namespace aslhip {
 
+// Threads per workgroup of the step kernel (each thread one cell).
+constexpr int STEP_BLOCK = 1024;
+
 // Step-kernel flags.
 enum : unsigned {
   STEP_F_RHO = 1u,           // write rho of computed cells
@@ -15,7 +18,7 @@ enum : unsigned {
 };
 
 // One fused collide+stream pass (AA pattern) over the ghost-inclusive block.
-// The launch is a 1-D grid of 256-cell chunks of the interior planes
+// The launch is a 1-D grid of STEP_BLOCK-cell chunks of the interior planes
 // x = 1..nx (rows y = 1..ny, every z including the two ghost columns).
 //
 // Links into cells of the six face layers are resolved from the cell's
@@ -32,7 +35,7 @@ struct StepArgs {
   int sx, sy;                 // strides of the layout: Sy*Sz and Sz
   int nx, ny, nz;             // interior size
   int len;                    // ghost-inclusive cell count
-  int bpp;                    // 256-cell chunks per plane
+  int bpp;                    // STEP_BLOCK-cell chunks per plane
   int nblocks;                // bpp * nx
   unsigned long long zmagic;  // ceil(2^40 / sy): row = (off * zmagic) >> 40
   int remap;                  // 1: XCD-contiguous chunk order (see xcd_remap)

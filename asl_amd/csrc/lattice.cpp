@@ -134,6 +134,10 @@ struct aslhip_lattice {
   bool fresh = false;        // no step since init
   bool bc_since_init = false;
   bool next_odd = true;
+  // a step requested with ASLHIP_STEP_DEFER that has not run yet
+  bool deferred = false;
+  bool deferred_raw = false;
+  bool deferred_first = false;
   // launch tuning
   int tune_remap = 1;
   // face rule: per face slot * 4 + mode of the winning slice BC, and values
@@ -233,7 +237,10 @@ struct PatHash {
   size_t operator()(const std::string &s) const { return std::hash<std::string>()(s); }
 };
 
-int finalize_impl(aslhip_lattice *h) {
+// allow_open: the next step reads ghost links raw (the first step after
+// initF with no BC executed before it), so links no BC fills are allowed; the
+// lattice is finalized again before any later step.
+int finalize_impl(aslhip_lattice *h, bool allow_open = false) {
   const Templ &t = h->t;
   const int Q = t.Q, len = h->len;
   std::vector<unsigned char> code(len, CODE_SKIP);
@@ -315,11 +322,13 @@ int finalize_impl(aslhip_lattice *h) {
               const int X = x + t.c[k][0], Y = y + t.c[k][1], Z = z + t.c[k][2];
               if (!interior(X, Y, Z)) continue;
               if (!computed[h->idx(X, Y, Z)] || resolve(X, Y, Z, k) != 0) continue;
-              if (!interior(x, y, z))
+              if (!interior(x, y, z)) {
+                if (allow_open) continue;
                 return fail(ASLHIP_EUNDEF,
                             "ASL ERROR: no boundary condition fills the ghost link (%d,%d,%d) -> (%d,%d,%d) "
                             "(ASL would stream uninitialised memory into the block)",
                             x - 1, y - 1, z - 1, X - 1, Y - 1, Z - 1);
+              }
               computed[s] = 1;
               changed = true;
               break;
@@ -493,7 +502,7 @@ int finalize_impl(aslhip_lattice *h) {
   HIPCHK(hipMemcpyAsync(h->vlbc, vlbc.data(), 256 * sizeof(VLinkBC), hipMemcpyHostToDevice, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   h->h_code.swap(code);
-  h->dirty = false;
+  h->dirty = allow_open;
   return 0;
 }
 
@@ -514,9 +523,11 @@ int run_vlinks(aslhip_lattice *h, bool after_odd) {
   return 0;
 }
 
-int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEvent_t> *ev) {
-  if (nsteps < 0) return fail(ASLHIP_EINVAL, "ASL ERROR: negative step count");
-  if (!h->initialized) return fail(ASLHIP_ESTATE, "ASL ERROR: LBGK::execute before initF");
+// Runs nsteps steps now.  raw_first: the first one reads ghost links raw
+// (no BC executed before it since init); macro_last: the last one writes rho/u.
+int run_steps(aslhip_lattice *h, int nsteps, bool first_after_init, bool raw_first, bool macro_last,
+              std::vector<hipEvent_t> *ev) {
+  if (nsteps <= 0) return 0;
   if (!h->pending.empty()) {
     if (h->pending != h->order) {
       h->order = h->pending;
@@ -525,17 +536,13 @@ int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEven
     h->pending.clear();
   }
   if (h->dirty) {
-    int r = finalize_impl(h);
+    int r = finalize_impl(h, raw_first && nsteps == 1);
     if (r) return r;
   }
-  if (nsteps == 0) return 0;
-  bool raw = false;
-  if (h->fresh) {
-    raw = !h->bc_since_init;
-    if (!raw) {
-      int r = run_vlinks(h, false);
-      if (r) return r;
-    }
+  if (first_after_init && !raw_first) {
+    // the BCs were applied to the initial state: velocity-map links too
+    int r = run_vlinks(h, false);
+    if (r) return r;
   }
   StepArgs a;
   for (int k = 0; k < 27; ++k) a.fd[k] = k < h->Q ? h->f + k * h->qs_bytes : nullptr;
@@ -554,7 +561,7 @@ int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEven
   a.ny = h->n[1];
   a.nz = h->n[2];
   a.len = h->len;
-  a.bpp = (h->n[1] * h->sy + 255) / 256;
+  a.bpp = (h->n[1] * h->sy + STEP_BLOCK - 1) / STEP_BLOCK;
   a.nblocks = a.bpp * h->n[0];
   a.zmagic = ((1ULL << 40) + (unsigned long long)h->sy - 1) / (unsigned long long)h->sy;
   a.remap = h->tune_remap;
@@ -568,8 +575,8 @@ int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEven
   a.w = relaxation(h);
   for (int s = 0; s < nsteps; ++s) {
     unsigned fl = h->compressible ? STEP_F_COMPRESSIBLE : 0u;
-    if (s == 0 && raw) fl |= STEP_F_RAW;
-    if (s == nsteps - 1 && (flags & ASLHIP_STEP_WRITE_MACRO)) {
+    if (s == 0 && raw_first) fl |= STEP_F_RAW;
+    if (s == nsteps - 1 && macro_last) {
       if (h->comp_rho) fl |= STEP_F_RHO;
       if (h->comp_vel) fl |= STEP_F_VEL;
     }
@@ -582,8 +589,37 @@ int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEven
     int r = run_vlinks(h, was_odd);
     if (r) return r;
   }
-  h->fresh = false;
   return 0;
+}
+
+// Runs a deferred step, writing rho and u (the host is about to look).
+int flush_deferred(aslhip_lattice *h) {
+  if (!h->deferred) return 0;
+  h->deferred = false;
+  return run_steps(h, 1, h->deferred_first, h->deferred_raw, true, nullptr);
+}
+
+int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEvent_t> *ev) {
+  if (nsteps < 0) return fail(ASLHIP_EINVAL, "ASL ERROR: negative step count");
+  if (!h->initialized) return fail(ASLHIP_ESTATE, "ASL ERROR: LBGK::execute before initF");
+  if (h->deferred) {  // a newer step follows: no need for its rho/u
+    h->deferred = false;
+    int r = run_steps(h, 1, h->deferred_first, h->deferred_raw, false, nullptr);
+    if (r) return r;
+  }
+  // the first step after initF reads raw ghost values unless a BC ran before it
+  const bool first = h->fresh && nsteps > 0;
+  const bool raw = first && !h->bc_since_init;
+  if (nsteps > 0) h->fresh = false;
+  if ((flags & ASLHIP_STEP_DEFER) && nsteps > 0) {
+    int r = run_steps(h, nsteps - 1, first, raw, false, ev);
+    if (r) return r;
+    h->deferred = true;
+    h->deferred_first = first && nsteps == 1;
+    h->deferred_raw = raw && nsteps == 1;
+    return 0;
+  }
+  return run_steps(h, nsteps, first, raw, (flags & ASLHIP_STEP_WRITE_MACRO) != 0, ev);
 }
 
 int copy_out(aslhip_lattice *h, const void *dev, void *host, size_t bytes) {
@@ -705,6 +741,7 @@ int aslhip_set_compute_macro(aslhip_lattice *h, int comp_rho, int comp_vel) {
 
 int aslhip_init_equilibrium(aslhip_lattice *h, double rho, const double u[3]) {
   if (int r = check(h)) return r;
+  if (int r = flush_deferred(h)) return r;
   InitArgs a;
   a.f = h->f;
   a.qs_bytes = h->qs_bytes;
@@ -725,6 +762,7 @@ int aslhip_init_equilibrium(aslhip_lattice *h, double rho, const double u[3]) {
 
 int aslhip_init_equilibrium_field(aslhip_lattice *h, const void *rho, const void *u) {
   if (int r = check(h)) return r;
+  if (int r = flush_deferred(h)) return r;
   if (!u) return fail(ASLHIP_EINVAL, "ASL ERROR: initF needs a velocity field");
   void *drho = nullptr, *du = nullptr;
   const size_t n = (size_t)h->len * h->melem;
@@ -826,22 +864,26 @@ int aslhip_step(aslhip_lattice *h, int nsteps, unsigned flags) {
 
 int aslhip_synchronize(aslhip_lattice *h) {
   if (int r = check(h)) return r;
+  if (int r = flush_deferred(h)) return r;
   HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
 }
 
 int aslhip_read_rho(aslhip_lattice *h, void *out) {
   if (int r = check(h)) return r;
+  if (int r = flush_deferred(h)) return r;
   return copy_out(h, h->rho, out, (size_t)h->len * h->melem);
 }
 
 int aslhip_read_velocity(aslhip_lattice *h, void *out) {
   if (int r = check(h)) return r;
+  if (int r = flush_deferred(h)) return r;
   return copy_out(h, h->u, out, (size_t)h->len * 3 * h->melem);
 }
 
 int aslhip_read_f(aslhip_lattice *h, void *out) {
   if (int r = check(h)) return r;
+  if (int r = flush_deferred(h)) return r;
   void *d = nullptr;
   const size_t bytes = (size_t)h->len * h->Q * h->melem;
   HIPCHK(hipMalloc(&d, bytes));
@@ -870,6 +912,7 @@ int aslhip_read_cell_types(aslhip_lattice *h, uint8_t *out) {
 
 int aslhip_reduce_conservation(aslhip_lattice *h, double out[5]) {
   if (int r = check(h)) return r;
+  if (int r = flush_deferred(h)) return r;
   if (h->dirty) {
     int r = aslhip_finalize(h);
     if (r) return r;
@@ -911,6 +954,8 @@ int aslhip_set_tuning(aslhip_lattice *h, int knob, int value) {
 
 int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags, double *ms_total, double *ms_kernel) {
   if (int r = check(h)) return r;
+  if (int r = flush_deferred(h)) return r;
+  flags &= ~ASLHIP_STEP_DEFER;
   if (nsteps < 1) return fail(ASLHIP_EINVAL, "ASL ERROR: time_steps needs at least one step");
   if (h->dirty || !h->pending.empty()) {
     int r = aslhip_finalize(h);

@@ -9,10 +9,10 @@ The names follow ASL's driver-facing API (``src/num/aslLBGK.h``,
 ``initAll`` / ``executeAll``.  Host views use ASL's ghost-inclusive,
 z-fastest layout, as numpy arrays of shape (nx+2, ny+2, nz+2).
 
-``Lattice`` is the thin handle wrapper underneath; ``LBGK.execute()`` keeps
-one step pending so that the step that precedes a host read is the only one
-that writes rho and u (what ASL's per-step writes leave visible is exactly
-that step's output).
+``Lattice`` is the thin handle wrapper underneath; ``LBGK.execute()`` asks
+the library to keep the step pending (``ASLHIP_STEP_DEFER``), so the step
+that precedes a host read is the only one that writes rho and u (what ASL's
+per-step writes leave visible is exactly that step's output).
 """
 import ctypes
 
@@ -21,7 +21,7 @@ import numpy as np
 from . import _lib
 from ._lib import (ASLError, BC_CONSTANT_PRESSURE, BC_CONSTANT_PRESSURE_VELOCITY, BC_CONSTANT_VELOCITY,
                    BC_NOSLIP, BC_NOSLIP_MAP, BC_PRESSURE_VELOCITY_MAP, BC_VELOCITY_MAP, BF16, F16, F32, F64,
-                   STEP_WRITE_MACRO, X0, XE, Y0, YE, Z0, ZE, call)
+                   STEP_DEFER, STEP_WRITE_MACRO, X0, XE, Y0, YE, Z0, ZE, call)
 
 _DT = {"f32": F32, "f64": F64, "bf16": BF16, "f16": F16, F32: F32, F64: F64, BF16: BF16, F16: F16}
 
@@ -108,8 +108,9 @@ class Lattice:
         call("aslhip_finalize", self.h)
 
     # stepping -------------------------------------------------------------
-    def step(self, nsteps=1, write_macro=True):
-        call("aslhip_step", self.h, int(nsteps), STEP_WRITE_MACRO if write_macro else 0)
+    def step(self, nsteps=1, write_macro=True, defer=False):
+        flags = (STEP_WRITE_MACRO if write_macro else 0) | (STEP_DEFER if defer else 0)
+        call("aslhip_step", self.h, int(nsteps), flags)
 
     def synchronize(self):
         call("aslhip_synchronize", self.h)
@@ -216,7 +217,6 @@ class LBGK(NumMethod):
         self.vectorTemplate = vt
         self.lattice = Lattice(block.size, vt.q, dtype=dtype, periodic=periodic, nu=nu)
         self.lattice.set_compute_macro(compRho, compVel)
-        self._pending = 0
 
     def init(self):
         pass
@@ -225,24 +225,15 @@ class LBGK(NumMethod):
         self.lattice.set_compressible(flag)
 
     def execute(self):
-        self._flush(write_macro=False)
-        self._pending = 1
-
-    def _flush(self, write_macro=True):
-        if self._pending:
-            self.lattice.step(self._pending, write_macro=write_macro)
-            self._pending = 0
+        self.lattice.step(1, defer=True)
 
     def getRho(self):
-        self._flush()
         return self.lattice.rho()
 
     def getVelocity(self):
-        self._flush()
         return self.lattice.velocity()
 
     def getF(self):
-        self._flush()
         return self.lattice.f()
 
     def getVectorTemplate(self):
@@ -260,7 +251,6 @@ class LBGKUtilities:
             rho, vel = 1.0, args[0]
         else:
             rho, vel = args
-        self.num._pending = 0
         self.num.lattice.init_equilibrium(rho, vel)
 
 

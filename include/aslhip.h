@@ -77,6 +77,9 @@ extern "C" {
 
 /* ---- step flags ------------------------------------------------------ */
 #define ASLHIP_STEP_WRITE_MACRO 1u /* last step of the call writes rho and u */
+#define ASLHIP_STEP_DEFER 2u       /* keep the last step pending: it runs, writing
+                                      rho and u, when the host next reads or
+                                      synchronises, or before the next step */
 
 typedef struct aslhip_lattice aslhip_lattice;
 

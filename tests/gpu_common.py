@@ -33,7 +33,12 @@ def run(G, O, steps, bcs, pre_execute=True):
         for g, o in bcs:
             G.execute_bc(g)
             O.execute_bc(o)
-    G.step(steps)
+        G.step(steps)
+    elif steps:
+        G.step(1)  # reads the initial ghost values raw
+        for g, _ in bcs:
+            G.execute_bc(g)
+        G.step(steps - 1)
     order = [o for _, o in bcs]
     if steps:
         O.run(steps, order)

@@ -38,9 +38,9 @@ class LinkModel:
         u = np.broadcast_to(np.asarray(u, dtype=float).reshape((3,) + (1,) * 3 if np.ndim(u) == 1 else (3,) + self.S), (3,) + self.S)
         self.f = np.stack([feq(rho, u, self.c[i]) for i in range(self.q)])
 
-    def add(self, kind, faces=0, mapv=None, p=0.0, u=(0.0, 0.0, 0.0)):
+    def add(self, kind, faces=0, mapv=None, p=0.0, u=(0.0, 0.0, 0.0), rot=None):
         self.bcs.append(dict(kind=kind, faces=faces, map=None if mapv is None else np.asarray(mapv, np.float32),
-                             p=float(p), u=np.asarray(u, float)))
+                             p=float(p), u=np.asarray(u, float), rot=None if rot is None else np.asarray(rot, float)))
         return len(self.bcs) - 1
 
     def _value(self, b, x, k, f):
@@ -56,6 +56,20 @@ class LinkModel:
             return b["p"]
         if kind == CONSTANT_PRESSURE_VELOCITY:
             return b["p"] + 3 * uc
+        if kind == VELOCITY_MAP:
+            # rho at the ghost node g from the populations its computation-node
+            # neighbours sent towards it, momentum = u_wall(pos_g) rho
+            g = np.array(x) - c[k]
+            num = den = 0.0
+            for j in range(1, self.q):
+                nb = tuple(g + c[j])
+                if b["map"][nb] > 0:
+                    num += self.w[j] * f[inv[j]][nb]
+                    den += self.w[j]
+            rho_g = num / den
+            pos = g - 1.0  # ghost-inclusive index -> position (dx = 1, block at 0)
+            uw = b["u"] if b["rot"] is None else np.cross(b["rot"][:3], pos - b["rot"][3:])
+            return rho_g + 3 * rho_g * float(np.dot(uw, c[k]))
         if kind == PRESSURE_VELOCITY_MAP:
             p, uu = b["p"], float(np.dot(b["u"], b["u"]))
             if self.compressible:

@@ -13,7 +13,7 @@ import pytest
 
 from asl_amd import geometry
 from linkmodel import (CONSTANT_PRESSURE, CONSTANT_PRESSURE_VELOCITY, CONSTANT_VELOCITY, NOSLIP, NOSLIP_MAP,
-                       PRESSURE_VELOCITY_MAP, LinkModel)
+                       PRESSURE_VELOCITY_MAP, VELOCITY_MAP, LinkModel)
 from oracle import OracleLattice, template
 
 ALL = 63
@@ -149,3 +149,22 @@ def test_channel_sphere_pressure_matches_link_model(compressible):
                            u0=(0.02, 0.0, 0.0))
     assert (~comp[1:-1, 1:-1, 1:-1]).sum() > 10  # the obstacles remove cells
     _compare(O, M, comp)
+
+
+@pytest.mark.parametrize("rotating", [False, True])
+def test_velocity_map_matches_link_model(rotating):
+    # BCVelocityMap (src/num/aslLBGKBC.cxx:290-348): rho at the ghost node
+    # from its computation-node neighbours, f_i(g) = rho_g + 3 rho_g u.c_i
+    size = (9, 8, 8)
+    body = geometry.normalize(geometry.sphere(size, 2.2, (4.0, 3.5, 3.5)))
+    rot = (0.0, 0.01, 0.02, 4.0, 3.5, 3.5) if rotating else None
+
+    def setup(L):
+        o = isinstance(L, OracleLattice)
+        return [L.add_slice_bc(NOSLIP, ALL) if o else L.add(NOSLIP, ALL),
+                L.add_map_bc(VELOCITY_MAP, body, u=(0.02, -0.01, 0.01), rot=rot) if o else
+                L.add(VELOCITY_MAP, mapv=body, u=(0.02, -0.01, 0.01), rot=rot)]
+
+    O, M, comp = _run_both(size, 19, 0.05, 10, setup, lambda ids: list(ids))
+    _compare(O, M, comp)
+    assert np.abs(O.velocity()[:, comp]).max() > 1e-3
