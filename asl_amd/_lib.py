@@ -16,6 +16,8 @@ BC_NOSLIP, BC_CONSTANT_PRESSURE, BC_CONSTANT_VELOCITY, BC_CONSTANT_PRESSURE_VELO
 BC_NOSLIP_MAP, BC_PRESSURE_VELOCITY_MAP, BC_VELOCITY_MAP = 5, 6, 7
 STEP_WRITE_MACRO = 1
 STEP_DEFER = 2
+STEP_GRAPH = 4
+GRAPH_CHUNK = 100
 CODE_FLUID, CODE_FLUID_BC, CODE_SKIP, CODE_GHOST = 0, 1, 2, 3
 TUNE_XCD_REMAP = 1
 
@@ -49,6 +51,10 @@ SIGNATURES = {
     "aslhip_stats": [_vp, ctypes.POINTER(ctypes.c_int64)],
     "aslhip_set_tuning": [_vp, ctypes.c_int, ctypes.c_int],
     "aslhip_time_steps": [_vp, ctypes.c_int, ctypes.c_uint, _dp, _dp],
+    "aslhip_batch_create": [ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int],
+    "aslhip_batch_destroy": [_vp],
+    "aslhip_batch_step": [_vp, ctypes.c_int, ctypes.c_uint],
+    "aslhip_batch_time_steps": [_vp, ctypes.c_int, ctypes.c_uint, _dp],
 }
 
 _LIB = None

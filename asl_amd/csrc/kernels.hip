@@ -190,13 +190,13 @@ __device__ __forceinline__ double face_val<double>(const StepArgs &a, int f, int
   return a.fval64[f][k];
 }
 
+// One cell of one step: chunk L of the lattice described by a.
 template <int Q, class S, bool ODD, bool PER, bool WIDE>
-__global__ __launch_bounds__(STEP_BLOCK) void k_step(const StepArgs a) {
+__device__ __forceinline__ void step_body(const StepArgs &a, const int L) {
   using CD = Codec<S>;
   using C = typename CD::C;
   using M = typename CD::M;
   constexpr unsigned ES = sizeof(S);
-  const int L = a.remap ? xcd_remap(blockIdx.x, a.nblocks) : static_cast<int>(blockIdx.x);
   const int plane = L / a.bpp;
   const int chunk = L - plane * a.bpp;
   const unsigned off = static_cast<unsigned>(chunk * STEP_BLOCK) + threadIdx.x;
@@ -323,13 +323,23 @@ __global__ __launch_bounds__(STEP_BLOCK) void k_step(const StepArgs a) {
   collide<Q, C>(v, C(a.w), (a.flags & STEP_F_COMPRESSIBLE) != 0, r, m);
 
   if (!live) return;
+  // Stores address every direction from fd[0] too (one scalar base pair
+  // instead of Q live pointers) unless the lattice is too large for it.
+  auto st = [&](int k) {
+    const int dir = ODD ? k : inv_dir<Q>(k);
+    const unsigned o = ODD ? nb(k, 1) : ib;
+    if constexpr (WIDE)
+      CD::st(a.fd[dir], o, v[k]);
+    else
+      CD::st(a.fd[0], static_cast<unsigned>(dir) * a.qs32 + o, v[k]);
+  };
   if (general) {
 #pragma unroll
     for (int k = 0; k < Q; ++k)
-      if (!((skipm >> k) & 1u)) CD::st(a.fd[ODD ? k : inv_dir<Q>(k)], ODD ? nb(k, 1) : ib, v[k]);
+      if (!((skipm >> k) & 1u)) st(k);
   } else {
 #pragma unroll
-    for (int k = 0; k < Q; ++k) CD::st(a.fd[ODD ? k : inv_dir<Q>(k)], ODD ? nb(k, 1) : ib, v[k]);
+    for (int k = 0; k < Q; ++k) st(k);
   }
   if (a.flags & STEP_F_RHO) static_cast<M *>(a.rho)[idx] = M(r);
   if (a.flags & STEP_F_VEL) {
@@ -338,6 +348,27 @@ __global__ __launch_bounds__(STEP_BLOCK) void k_step(const StepArgs a) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) u[d * a.len + idx] = M(comp ? m[d] / r : m[d]);
   }
+}
+
+// The arguments live in device memory, not in the kernel-argument segment:
+// loads from the argument segment are invariant, and the compiler hoists all
+// of them (27 direction pointers, face tables) into scalar registers, runs
+// out of them and spills into vector registers (D3Q19 FP32: 103 VGPRs
+// instead of 70).  Loads through a plain pointer stay where they are used.
+template <int Q, class S, bool ODD, bool PER, bool WIDE>
+__global__ __launch_bounds__(STEP_BLOCK) void k_step(const StepArgs *args) {
+  const StepArgs &a = *args;
+  step_body<Q, S, ODD, PER, WIDE>(a, a.remap ? xcd_remap(blockIdx.x, a.nblocks) : static_cast<int>(blockIdx.x));
+}
+
+// Many lattices in one launch: block b works on lattice lat[b], chunk
+// b - first[lat[b]] of it; descriptors live in device memory.
+template <int Q, class S, bool ODD, bool PER, bool WIDE>
+__global__ __launch_bounds__(STEP_BLOCK) void k_step_batch(const StepArgs *desc, const unsigned short *lat,
+                                                           const int *first) {
+  const int l = lat[blockIdx.x];
+  const StepArgs &a = desc[l];
+  step_body<Q, S, ODD, PER, WIDE>(a, static_cast<int>(blockIdx.x) - first[l]);
 }
 
 // Boundary-link kernel: ASL's BCVelocityMap (src/num/aslLBGKBC.cxx:290-348).
@@ -508,17 +539,27 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_final(const double *partial, i
 }
 
 template <int Q, class S, bool ODD, bool PER>
-hipError_t step_w(bool wide, const StepArgs &a, hipStream_t s) {
+hipError_t step_w(bool wide, const StepLaunch &a, hipStream_t s) {
   const dim3 grid(a.nblocks), block(STEP_BLOCK);
   if (wide)
-    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true>), grid, block, 0, s, a.args);
   else
-    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, false>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, false>), grid, block, 0, s, a.args);
+  return hipGetLastError();
+}
+
+template <int Q, class S, bool ODD, bool PER>
+hipError_t batch_w(bool wide, const BatchLaunch &b, hipStream_t s) {
+  const dim3 grid(b.nblocks), block(STEP_BLOCK);
+  if (wide)
+    hipLaunchKernelGGL((k_step_batch<Q, S, ODD, PER, true>), grid, block, 0, s, b.desc, b.lat, b.first);
+  else
+    hipLaunchKernelGGL((k_step_batch<Q, S, ODD, PER, false>), grid, block, 0, s, b.desc, b.lat, b.first);
   return hipGetLastError();
 }
 
 template <int Q, class S>
-hipError_t step_q_s(bool odd, bool per, bool wide, const StepArgs &a, hipStream_t s) {
+hipError_t step_q_s(bool odd, bool per, bool wide, const StepLaunch &a, hipStream_t s) {
   if (per) return odd ? step_w<Q, S, true, true>(wide, a, s) : step_w<Q, S, false, true>(wide, a, s);
   return odd ? step_w<Q, S, true, false>(wide, a, s) : step_w<Q, S, false, false>(wide, a, s);
 }
@@ -544,8 +585,15 @@ hipError_t dispatch(int q, int dtype, Args &&...args) {
 
 template <int Q, class S>
 struct StepF {
-  static hipError_t run(bool odd, bool per, bool wide, const StepArgs &a, hipStream_t s) {
+  static hipError_t run(bool odd, bool per, bool wide, const StepLaunch &a, hipStream_t s) {
     return step_q_s<Q, S>(odd, per, wide, a, s);
+  }
+};
+template <int Q, class S>
+struct BatchF {
+  static hipError_t run(bool odd, bool per, bool wide, const BatchLaunch &b, hipStream_t s) {
+    if (per) return odd ? batch_w<Q, S, true, true>(wide, b, s) : batch_w<Q, S, false, true>(wide, b, s);
+    return odd ? batch_w<Q, S, true, false>(wide, b, s) : batch_w<Q, S, false, false>(wide, b, s);
   }
 };
 template <int Q, class S>
@@ -587,8 +635,12 @@ int reduce_blocks(int len) {
   return nb < RED_MAX_BLOCKS ? (nb < 1 ? 1 : nb) : RED_MAX_BLOCKS;
 }
 
-hipError_t launch_step(int q, int dtype, bool odd, bool periodic, bool wide, const StepArgs &a, hipStream_t s) {
+hipError_t launch_step(int q, int dtype, bool odd, bool periodic, bool wide, const StepLaunch &a, hipStream_t s) {
   return dispatch<StepF>(q, dtype, odd, periodic, wide, a, s);
+}
+hipError_t launch_step_batch(int q, int dtype, bool odd, bool periodic, bool wide, const BatchLaunch &b,
+                             hipStream_t s) {
+  return dispatch<BatchF>(q, dtype, odd, periodic, wide, b, s);
 }
 hipError_t launch_vlinks(int q, int dtype, const VLinkArgs &a, hipStream_t s) {
   return dispatch<VLinkF>(q, dtype, a, s);

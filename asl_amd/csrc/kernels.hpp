@@ -52,6 +52,16 @@ struct StepArgs {
   unsigned flags;
 };
 
+// Several lattices advanced by one launch of the step kernel: desc[l] is
+// lattice l's StepArgs (device memory), workgroup b works on lattice lat[b],
+// chunk b - first[lat[b]].
+struct BatchLaunch {
+  const StepArgs *desc;
+  const unsigned short *lat;
+  const int *first;
+  int nblocks;
+};
+
 // Boundary-link kernel entries (velocity map BCs).
 struct VLink {
   int g;              // ghost cell (linear index)
@@ -103,8 +113,16 @@ struct ReduceArgs {
   double *partial;  // 5 per block
 };
 
+// One step of one lattice: args points to a StepArgs in device memory.
+struct StepLaunch {
+  const StepArgs *args;
+  int nblocks;
+};
+
 // wide: the populations span more than 4 GiB (64-bit per-lane addresses).
-hipError_t launch_step(int q, int dtype, bool odd, bool periodic, bool wide, const StepArgs &a, hipStream_t s);
+hipError_t launch_step(int q, int dtype, bool odd, bool periodic, bool wide, const StepLaunch &a, hipStream_t s);
+hipError_t launch_step_batch(int q, int dtype, bool odd, bool periodic, bool wide, const BatchLaunch &b,
+                             hipStream_t s);
 hipError_t launch_vlinks(int q, int dtype, const VLinkArgs &a, hipStream_t s);
 hipError_t launch_init(int q, int dtype, const InitArgs &a, hipStream_t s);
 hipError_t launch_unpermute(int q, int dtype, const UnpermuteArgs &a, hipStream_t s);

@@ -121,6 +121,8 @@ struct aslhip_lattice {
   VLink *vl = nullptr;
   VLinkBC *vlbc = nullptr;
   double *red_partial = nullptr, *red_out = nullptr;
+  StepArgs *d_args = nullptr;  // [16]: step arguments for each value of StepArgs::flags
+  std::vector<StepArgs> h_args;  // host copy of d_args (empty: not uploaded yet)
   int nvl = 0;
   // parameters
   double nu = 0.1;
@@ -159,7 +161,8 @@ int check(aslhip_lattice *h) {
 }
 
 void free_all(aslhip_lattice *h) {
-  void *ptrs[] = {h->f, h->rho, h->u, h->code, h->pat, h->rec, h->mode, h->val, h->vl, h->vlbc, h->red_partial, h->red_out};
+  void *ptrs[] = {h->f,  h->rho,  h->u,           h->code,         h->pat, h->rec, h->mode,
+                  h->val, h->vl, h->vlbc, h->red_partial, h->red_out, h->d_args};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -523,31 +526,14 @@ int run_vlinks(aslhip_lattice *h, bool after_odd) {
   return 0;
 }
 
-// Runs nsteps steps now.  raw_first: the first one reads ghost links raw
-// (no BC executed before it since init); macro_last: the last one writes rho/u.
-int run_steps(aslhip_lattice *h, int nsteps, bool first_after_init, bool raw_first, bool macro_last,
-              std::vector<hipEvent_t> *ev) {
-  if (nsteps <= 0) return 0;
-  if (!h->pending.empty()) {
-    if (h->pending != h->order) {
-      h->order = h->pending;
-      h->dirty = true;
-    }
-    h->pending.clear();
-  }
-  if (h->dirty) {
-    int r = finalize_impl(h, raw_first && nsteps == 1);
-    if (r) return r;
-  }
-  if (first_after_init && !raw_first) {
-    // the BCs were applied to the initial state: velocity-map links too
-    int r = run_vlinks(h, false);
-    if (r) return r;
-  }
+bool is_wide(const aslhip_lattice *h) { return h->qs_bytes * h->Q > 0xffffffffLL; }
+
+// Kernel arguments of one step of h (flags filled in by the caller).
+StepArgs step_args(const aslhip_lattice *h) {
   StepArgs a;
+  std::memset(&a, 0, sizeof a);  // compared bytewise by upload_args
   for (int k = 0; k < 27; ++k) a.fd[k] = k < h->Q ? h->f + k * h->qs_bytes : nullptr;
-  const bool wide = h->qs_bytes * h->Q > 0xffffffffLL;
-  a.qs32 = wide ? 0u : static_cast<unsigned>(h->qs_bytes);
+  a.qs32 = is_wide(h) ? 0u : static_cast<unsigned>(h->qs_bytes);
   for (int fc = 0; fc < 6; ++fc) {
     a.fface[fc] = h->fface[fc];
     for (int k = 0; k < 27; ++k) {
@@ -573,6 +559,54 @@ int run_steps(aslhip_lattice *h, int nsteps, bool first_after_init, bool raw_fir
   a.rho = h->rho;
   a.u = h->u;
   a.w = relaxation(h);
+  a.flags = h->compressible ? STEP_F_COMPRESSIBLE : 0u;
+  return a;
+}
+
+// Runs nsteps steps now.  raw_first: the first one reads ghost links raw
+// (no BC executed before it since init); macro_last: the last one writes rho/u.
+// The execute order of the BCs since the last step becomes the active one;
+// the tables are rebuilt if anything changed.
+int prepare(aslhip_lattice *h, bool allow_open) {
+  if (!h->pending.empty()) {
+    if (h->pending != h->order) {
+      h->order = h->pending;
+      h->dirty = true;
+    }
+    h->pending.clear();
+  }
+  return h->dirty ? finalize_impl(h, allow_open) : 0;
+}
+
+// Brings the device copies of the step arguments (one per flags value) up to
+// date; they change with the BC tables, viscosity, compressibility and tuning.
+int upload_args(aslhip_lattice *h) {
+  const StepArgs base = step_args(h);
+  if (!h->h_args.empty()) {
+    if (std::memcmp(&base, &h->h_args[base.flags], sizeof base) == 0) return 0;
+    // the previous upload reads h_args: let it finish before overwriting
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  h->h_args.assign(16, base);
+  for (unsigned fl = 0; fl < 16; ++fl) h->h_args[fl].flags = fl;
+  // stream-ordered: steps queued before still see the previous arguments
+  HIPCHK(hipMemcpyAsync(h->d_args, h->h_args.data(), 16 * sizeof(StepArgs), hipMemcpyHostToDevice, h->stream));
+  return 0;
+}
+
+int run_steps(aslhip_lattice *h, int nsteps, bool first_after_init, bool raw_first, bool macro_last,
+              std::vector<hipEvent_t> *ev) {
+  if (nsteps <= 0) return 0;
+  if (int r = prepare(h, raw_first && nsteps == 1)) return r;
+  if (first_after_init && !raw_first) {
+    // the BCs were applied to the initial state: velocity-map links too
+    int r = run_vlinks(h, false);
+    if (r) return r;
+  }
+  if (int r = upload_args(h)) return r;
+  StepLaunch a;
+  a.nblocks = h->h_args[0].nblocks;
+  const bool wide = is_wide(h);
   for (int s = 0; s < nsteps; ++s) {
     unsigned fl = h->compressible ? STEP_F_COMPRESSIBLE : 0u;
     if (s == 0 && raw_first) fl |= STEP_F_RAW;
@@ -580,7 +614,7 @@ int run_steps(aslhip_lattice *h, int nsteps, bool first_after_init, bool raw_fir
       if (h->comp_rho) fl |= STEP_F_RHO;
       if (h->comp_vel) fl |= STEP_F_VEL;
     }
-    a.flags = fl;
+    a.args = h->d_args + fl;
     if (ev) HIPCHK(hipEventRecord((*ev)[2 * s], h->stream));
     HIPCHK(launch_step(h->Q, h->dtype, h->next_odd, h->periodic, wide, a, h->stream));
     if (ev) HIPCHK(hipEventRecord((*ev)[2 * s + 1], h->stream));
@@ -698,6 +732,7 @@ int aslhip_lattice_create(aslhip_lattice **out, int nx, int ny, int nz, int q, i
   if (e == hipSuccess) e = hipMalloc(&h->mode, 256);
   if (e == hipSuccess) e = hipMalloc(&h->val, 256 * (size_t)q * h->celem);
   if (e == hipSuccess) e = hipMalloc(&h->vlbc, 256 * sizeof(VLinkBC));
+  if (e == hipSuccess) e = hipMalloc(&h->d_args, 16 * sizeof(StepArgs));
   if (e == hipSuccess) e = hipMalloc(&h->red_partial, (size_t)reduce_blocks((int)len) * 5 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->red_out, 5 * sizeof(double));
   if (e != hipSuccess) {
@@ -983,6 +1018,199 @@ int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags, double *ms_
     if (ms_kernel) *ms_kernel = ksum;
   }
   for (auto &e : ev) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  return r;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches of lattices
+// ---------------------------------------------------------------------------
+struct aslhip_batch {
+  std::vector<aslhip_lattice *> lat;
+  int Q = 19, dtype = ASLHIP_F32;
+  bool periodic = false, wide = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  StepArgs *desc = nullptr;  // [2][n]: plain steps, then steps writing rho/u
+  unsigned short *blat = nullptr;
+  int *first = nullptr;
+  int nblocks = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // chunks starting with an even / odd step
+  std::vector<StepArgs> hdesc;                    // host copy of desc (source of the async upload)
+};
+
+namespace {
+
+void batch_free(aslhip_batch *b) {
+  for (auto &g : b->graph)
+    if (g) (void)hipGraphExecDestroy(g);
+  void *ptrs[] = {b->desc, b->blat, b->first};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+int batch_launch(aslhip_batch *b, bool odd, bool macro) {
+  BatchLaunch L;
+  L.desc = b->desc + (macro ? b->lat.size() : 0);
+  L.lat = b->blat;
+  L.first = b->first;
+  L.nblocks = b->nblocks;
+  HIPCHK(launch_step_batch(b->Q, b->dtype, odd, b->periodic, b->wide, L, b->stream));
+  return 0;
+}
+
+int batch_graph(aslhip_batch *b, bool odd) {
+  hipGraphExec_t &g = b->graph[odd ? 1 : 0];
+  if (g) return 0;
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  bool o = odd;
+  int r = 0;
+  for (int s = 0; s < ASLHIP_GRAPH_CHUNK && !r; ++s, o = !o) r = batch_launch(b, o, false);
+  hipError_t e = hipStreamEndCapture(b->stream, &graph);
+  if (r) return r;
+  if (e != hipSuccess) return fail(ASLHIP_EHIP, "ASL ERROR: graph capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(ASLHIP_EHIP, "ASL ERROR: graph instantiate: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int batch_step_impl(aslhip_batch *b, int nsteps, unsigned flags) {
+  if (nsteps < 0) return fail(ASLHIP_EINVAL, "ASL ERROR: negative step count");
+  if (nsteps == 0) return 0;
+  // Lattices fresh from initF take their first step on their own (it may
+  // read ghost links raw); all of them then share one parity.
+  bool any_fresh = false;
+  for (auto *h : b->lat) {
+    if (!h->initialized) return fail(ASLHIP_ESTATE, "ASL ERROR: batch lattice without initF");
+    if (int r = flush_deferred(h)) return r;
+    any_fresh = any_fresh || h->fresh;
+  }
+  if (any_fresh) {
+    for (auto *h : b->lat)
+      if (int r = step_impl(h, 1, (nsteps == 1 && (flags & ASLHIP_STEP_WRITE_MACRO)) ? ASLHIP_STEP_WRITE_MACRO : 0u,
+                            nullptr))
+        return r;
+    if (--nsteps == 0) return 0;
+  }
+  const size_t n = b->lat.size();
+  // the previous upload may still be in flight from this buffer
+  HIPCHK(hipStreamSynchronize(b->stream));
+  std::vector<StepArgs> &desc = b->hdesc;
+  desc.resize(2 * n);
+  const bool odd = b->lat[0]->next_odd;
+  for (size_t l = 0; l < n; ++l) {
+    aslhip_lattice *h = b->lat[l];
+    if (int r = prepare(h, false)) return r;
+    if (h->nvl) return fail(ASLHIP_EINVAL, "ASL ERROR: velocity map BCs are not supported in batches");
+    if (h->next_odd != odd) return fail(ASLHIP_ESTATE, "ASL ERROR: batch lattices are at different step parities");
+    desc[l] = step_args(h);
+    desc[n + l] = desc[l];
+    if (h->comp_rho) desc[n + l].flags |= STEP_F_RHO;
+    if (h->comp_vel) desc[n + l].flags |= STEP_F_VEL;
+    // work queued on the lattice's own stream comes first
+    HIPCHK(hipEventRecord(b->done, h->stream));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->done, 0));
+  }
+  HIPCHK(hipMemcpyAsync(b->desc, desc.data(), desc.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
+  const bool macro = (flags & ASLHIP_STEP_WRITE_MACRO) != 0;
+  int left = nsteps;
+  bool o = odd;
+  if (flags & ASLHIP_STEP_GRAPH) {
+    while (left - (macro ? 1 : 0) >= ASLHIP_GRAPH_CHUNK) {
+      if (int r = batch_graph(b, o)) return r;
+      HIPCHK(hipGraphLaunch(b->graph[o ? 1 : 0], b->stream));
+      left -= ASLHIP_GRAPH_CHUNK;  // an even count: the parity is unchanged
+    }
+  }
+  for (; left > 0; --left, o = !o)
+    if (int r = batch_launch(b, o, macro && left == 1)) return r;
+  for (auto *h : b->lat) h->next_odd = (nsteps % 2) ? !odd : odd;
+  // the lattices' own streams see the result in order
+  HIPCHK(hipEventRecord(b->done, b->stream));
+  for (auto *h : b->lat) HIPCHK(hipStreamWaitEvent(h->stream, b->done, 0));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aslhip_batch_create(aslhip_batch **out, aslhip_lattice *const *lattices, int n) {
+  if (!out || !lattices || n < 1) return fail(ASLHIP_EINVAL, "ASL ERROR: batch needs at least one lattice");
+  *out = nullptr;
+  if (n > 65535) return fail(ASLHIP_EINVAL, "ASL ERROR: at most 65535 lattices per batch");
+  aslhip_batch *b = new aslhip_batch();
+  b->lat.assign(lattices, lattices + n);
+  b->Q = lattices[0]->Q;
+  b->dtype = lattices[0]->dtype;
+  b->periodic = lattices[0]->periodic;
+  b->wide = is_wide(lattices[0]);
+  std::vector<int> first(n);
+  std::vector<unsigned short> blat;
+  for (int l = 0; l < n; ++l) {
+    aslhip_lattice *h = lattices[l];
+    if (!h || h->Q != b->Q || h->dtype != b->dtype || h->periodic != b->periodic || is_wide(h) != b->wide) {
+      delete b;
+      return fail(ASLHIP_EINVAL, "ASL ERROR: batch lattices must share template, population type and periodicity");
+    }
+    const StepArgs a = step_args(h);
+    first[l] = static_cast<int>(blat.size());
+    blat.insert(blat.end(), a.nblocks, static_cast<unsigned short>(l));
+  }
+  b->nblocks = static_cast<int>(blat.size());
+  hipError_t e = hipStreamCreate(&b->stream);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipMalloc(&b->desc, 2 * (size_t)n * sizeof(StepArgs));
+  if (e == hipSuccess) e = hipMalloc(&b->blat, blat.size() * sizeof(unsigned short));
+  if (e == hipSuccess) e = hipMalloc(&b->first, (size_t)n * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(b->blat, blat.data(), blat.size() * sizeof(unsigned short), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->first, first.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    batch_free(b);
+    delete b;
+    return fail(ASLHIP_EHIP, "ASL ERROR: batch set-up: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return 0;
+}
+
+int aslhip_batch_destroy(aslhip_batch *b) {
+  if (!b) return 0;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_free(b);
+  delete b;
+  return 0;
+}
+
+int aslhip_batch_step(aslhip_batch *b, int nsteps, unsigned flags) {
+  if (!b) return fail(ASLHIP_EINVAL, "ASL ERROR: null batch handle");
+  return batch_step_impl(b, nsteps, flags);
+}
+
+int aslhip_batch_time_steps(aslhip_batch *b, int nsteps, unsigned flags, double *ms_total) {
+  if (!b) return fail(ASLHIP_EINVAL, "ASL ERROR: null batch handle");
+  hipEvent_t t0, t1;
+  HIPCHK(hipEventCreate(&t0));
+  HIPCHK(hipEventCreate(&t1));
+  // start from an idle batch stream (previous work of the lattices included)
+  for (auto *h : b->lat) {
+    HIPCHK(hipEventRecord(b->done, h->stream));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->done, 0));
+  }
+  HIPCHK(hipEventRecord(t0, b->stream));
+  int r = batch_step_impl(b, nsteps, flags);
+  HIPCHK(hipEventRecord(t1, b->stream));
+  HIPCHK(hipEventSynchronize(t1));
+  float ms = 0.f;
+  if (r == 0) HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+  if (ms_total) *ms_total = ms;
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   return r;

@@ -21,7 +21,7 @@ import numpy as np
 from . import _lib
 from ._lib import (ASLError, BC_CONSTANT_PRESSURE, BC_CONSTANT_PRESSURE_VELOCITY, BC_CONSTANT_VELOCITY,
                    BC_NOSLIP, BC_NOSLIP_MAP, BC_PRESSURE_VELOCITY_MAP, BC_VELOCITY_MAP, BF16, F16, F32, F64,
-                   STEP_DEFER, STEP_WRITE_MACRO, X0, XE, Y0, YE, Z0, ZE, call)
+                   STEP_DEFER, STEP_GRAPH, STEP_WRITE_MACRO, X0, XE, Y0, YE, Z0, ZE, call)
 
 _DT = {"f32": F32, "f64": F64, "bf16": BF16, "f16": F16, F32: F32, F64: F64, BF16: BF16, F16: F16}
 
@@ -155,6 +155,38 @@ class Lattice:
         out = (ctypes.c_int64 * 4)()
         call("aslhip_stats", self.h, out)
         return {"counted": out[0], "computed": out[1], "boundary": out[2], "patterns": out[3]}
+
+
+class Batch:
+    """Independent lattices advanced together: one step-kernel launch per step for all of them."""
+
+    def __init__(self, lattices):
+        self.lattices = list(lattices)
+        arr = (ctypes.c_void_p * len(self.lattices))(*[l.h for l in self.lattices])
+        h = ctypes.c_void_p()
+        call("aslhip_batch_create", ctypes.byref(h), arr, len(self.lattices))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load_library().aslhip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nsteps=1, write_macro=True, graph=False):
+        flags = (STEP_WRITE_MACRO if write_macro else 0) | (STEP_GRAPH if graph else 0)
+        call("aslhip_batch_step", self.h, int(nsteps), flags)
+
+    def time_steps(self, nsteps, graph=True, write_macro=False):
+        ms = ctypes.c_double(0)
+        flags = (STEP_WRITE_MACRO if write_macro else 0) | (STEP_GRAPH if graph else 0)
+        call("aslhip_batch_time_steps", self.h, int(nsteps), flags, ctypes.byref(ms))
+        return ms.value
 
 
 # ---------------------------------------------------------------------------

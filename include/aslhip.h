@@ -196,6 +196,24 @@ int aslhip_set_tuning(aslhip_lattice *h, int knob, int value);
 int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags,
                       double *ms_total, double *ms_kernel);
 
+/* ---- batches of independent lattices ---------------------------------
+ * Extension (ASL advances each lattice with its own blocking launches):
+ * many lattices of uneven sizes, each with its own viscosity and BCs, advance
+ * together, one launch of the step kernel per time step for all of them.
+ * All lattices of a batch share the template, population type and
+ * periodicity, and have no velocity-map BCs.  ASLHIP_STEP_GRAPH replays
+ * hipGraph-captured chunks of ASLHIP_GRAPH_CHUNK steps. */
+#define ASLHIP_STEP_GRAPH 4u
+#define ASLHIP_GRAPH_CHUNK 100
+typedef struct aslhip_batch aslhip_batch;
+int aslhip_batch_create(aslhip_batch **out, aslhip_lattice *const *lattices, int n);
+int aslhip_batch_destroy(aslhip_batch *b);
+/* nsteps steps of every lattice (flags: ASLHIP_STEP_WRITE_MACRO,
+ * ASLHIP_STEP_GRAPH).  Lattices' own reads see the result in order. */
+int aslhip_batch_step(aslhip_batch *b, int nsteps, unsigned flags);
+/* As aslhip_batch_step, timed with HIP events on the batch's stream. */
+int aslhip_batch_time_steps(aslhip_batch *b, int nsteps, unsigned flags, double *ms_total);
+
 #ifdef __cplusplus
 }
 #endif
