@@ -104,8 +104,16 @@ void LBGK::setViscosity(Param nu) {
 
 double LBGK::getViscosity(unsigned int) { return nu_; }
 
-void LBGK::setOmega(Param) {
-  errorMessage("LBGK::setOmega: the Coriolis term is not available in the fused gfx950 kernel");
+// src/num/aslLBGK.cxx:252-255; the term itself: :207-208 (crossProduct
+// needs a 3-component omega in 3D, aclVectorOfElementsOperations.cxx:514-535).
+void LBGK::setOmega(Param w) {
+  if (w.size() == 0) {
+    check_abi(aslhip_set_omega(h_, nullptr));
+    return;
+  }
+  if (w.size() != 3) errorMessage("crossProduct - two VectorOfElements have different sizes");
+  const double om[3] = {w[0], w[1], w[2]};
+  check_abi(aslhip_set_omega(h_, om));
 }
 
 void LBGK::setVectorTemplate(const VectorTemplate *vT) {

@@ -121,7 +121,8 @@ __device__ __forceinline__ C dot_c(const C *u, int i) {
 // same value ASL computes twice.  On return v holds the post-collision
 // populations, r the density and m the momentum.
 template <int Q, class C>
-__device__ __forceinline__ void collide(C (&v)[Q], C w, bool compressible, C &r_o, C (&m_o)[3]) {
+__device__ __forceinline__ void collide(C (&v)[Q], C w, bool compressible, bool omega, const C (&om)[3], C &r_o,
+                                        C (&m_o)[3]) {
   using LT = Lat<Q>;
   C m[3];
 #pragma unroll
@@ -140,6 +141,15 @@ __device__ __forceinline__ void collide(C (&v)[Q], C w, bool compressible, C &r_
   C r = v[0] * C(LT::w(0));
 #pragma unroll
   for (int j = 1; j < Q; ++j) r = fmaT(v[j], C(LT::w(j)), r);
+  // LBGK::setOmega (src/num/aslLBGK.cxx:207-208): momentum += (2/w) omega x
+  // momentum, one component after the other, each seeing the ones before it
+  // (ASL emits one statement per component).
+  if (omega) {
+    const C t = C(2.) / w;
+    m[0] = fmaT(t, fmaT(om[1], m[2], -(om[2] * m[1])), m[0]);
+    m[1] = fmaT(t, fmaT(om[2], m[0], -(om[0] * m[2])), m[1]);
+    m[2] = fmaT(t, fmaT(om[0], m[1], -(om[1] * m[0])), m[2]);
+  }
   C mm = (C(1.5) * m[0]) * m[0];
   mm = fmaT(C(1.5) * m[1], m[1], mm);
   mm = fmaT(C(1.5) * m[2], m[2], mm);
@@ -320,7 +330,13 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L) {
   }
 
   C r, m[3];
-  collide<Q, C>(v, C(a.w), (a.flags & STEP_F_COMPRESSIBLE) != 0, r, m);
+  const bool omega = (a.flags & STEP_F_OMEGA) != 0;
+  C om[3] = {C(0), C(0), C(0)};
+  if (omega) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) om[d] = sizeof(C) == 4 ? C(a.om32[d]) : C(a.om64[d]);
+  }
+  collide<Q, C>(v, C(a.w), (a.flags & STEP_F_COMPRESSIBLE) != 0, omega, om, r, m);
 
   if (!live) return;
   // Stores address every direction from fd[0] too (one scalar base pair

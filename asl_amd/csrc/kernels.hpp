@@ -15,6 +15,8 @@ enum : unsigned {
   STEP_F_VEL = 2u,           // write u of computed cells
   STEP_F_RAW = 4u,           // read ghost links raw (no BC resolution)
   STEP_F_COMPRESSIBLE = 8u,  // compressible equilibrium (ASL default)
+  STEP_F_OMEGA = 16u,        // rotating frame: momentum += (2/w) omega x momentum
+  STEP_F_COUNT = 32u,        // number of flag combinations
 };
 
 // One fused collide+stream pass (AA pattern) over the ghost-inclusive block.
@@ -42,6 +44,8 @@ struct StepArgs {
   int fface[6];               // per face: slot * 4 + mode of its winning slice BC
   float fval32[6][27];        // per face and direction: addend / constant (FP32 lattices)
   double fval64[6][27];       // the same for FP64 lattices
+  float om32[3];              // angular velocity of the frame (STEP_F_OMEGA)
+  double om64[3];
   const unsigned char *code;  // packed cell types
   const unsigned short *pat;  // link pattern of CODE_FLUID_BC / CODE_GHOST cells
   const LinkRec *rec;         // link patterns

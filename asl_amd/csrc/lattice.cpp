@@ -121,12 +121,17 @@ struct aslhip_lattice {
   VLink *vl = nullptr;
   VLinkBC *vlbc = nullptr;
   double *red_partial = nullptr, *red_out = nullptr;
-  StepArgs *d_args = nullptr;  // [16]: step arguments for each value of StepArgs::flags
+  StepArgs *d_args = nullptr;  // [STEP_F_COUNT]: step arguments for each value of StepArgs::flags
   std::vector<StepArgs> h_args;  // host copy of d_args (empty: not uploaded yet)
+  // hipGraph chunks of ASLHIP_GRAPH_CHUNK plain steps starting with an even /
+  // odd step (the velocity-map link fills included); rebuilt after finalize
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
   int nvl = 0;
   // parameters
   double nu = 0.1;
   bool compressible = true, comp_rho = true, comp_vel = true;
+  bool has_omega = false;
+  double omega[3] = {0., 0., 0.};
   double dx = 1., pos0[3] = {0., 0., 0.};
   // BCs and order
   std::vector<BCDesc> bcs;
@@ -160,7 +165,16 @@ int check(aslhip_lattice *h) {
   return 0;
 }
 
+void drop_graphs(aslhip_lattice *h) {
+  for (auto &g : h->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
 void free_all(aslhip_lattice *h) {
+  drop_graphs(h);
   void *ptrs[] = {h->f,  h->rho,  h->u,           h->code,         h->pat, h->rec, h->mode,
                   h->val, h->vl, h->vlbc, h->red_partial, h->red_out, h->d_args};
   for (void *p : ptrs)
@@ -170,6 +184,11 @@ void free_all(aslhip_lattice *h) {
 
 // Relaxation rate w = 1/(3 nu + 1/2) in the compute type
 // (src/num/aslLBGK.cxx:203).
+// StepArgs::flags bits fixed by the lattice's parameters.
+unsigned base_flags(const aslhip_lattice *h) {
+  return (h->compressible ? STEP_F_COMPRESSIBLE : 0u) | (h->has_omega ? STEP_F_OMEGA : 0u);
+}
+
 double relaxation(const aslhip_lattice *h) {
   if (h->dtype == ASLHIP_F64) return 1. / (3. * h->nu + .5);
   const float nu = static_cast<float>(h->nu);
@@ -244,6 +263,7 @@ struct PatHash {
 // initF with no BC executed before it), so links no BC fills are allowed; the
 // lattice is finalized again before any later step.
 int finalize_impl(aslhip_lattice *h, bool allow_open = false) {
+  drop_graphs(h);  // they hold the link tables replaced below
   const Templ &t = h->t;
   const int Q = t.Q, len = h->len;
   std::vector<unsigned char> code(len, CODE_SKIP);
@@ -559,7 +579,11 @@ StepArgs step_args(const aslhip_lattice *h) {
   a.rho = h->rho;
   a.u = h->u;
   a.w = relaxation(h);
-  a.flags = h->compressible ? STEP_F_COMPRESSIBLE : 0u;
+  for (int d = 0; d < 3; ++d) {
+    a.om32[d] = static_cast<float>(h->omega[d]);
+    a.om64[d] = h->omega[d];
+  }
+  a.flags = base_flags(h);
   return a;
 }
 
@@ -587,15 +611,54 @@ int upload_args(aslhip_lattice *h) {
     // the previous upload reads h_args: let it finish before overwriting
     HIPCHK(hipStreamSynchronize(h->stream));
   }
-  h->h_args.assign(16, base);
-  for (unsigned fl = 0; fl < 16; ++fl) h->h_args[fl].flags = fl;
+  h->h_args.assign(STEP_F_COUNT, base);
+  for (unsigned fl = 0; fl < STEP_F_COUNT; ++fl) h->h_args[fl].flags = fl;
   // stream-ordered: steps queued before still see the previous arguments
-  HIPCHK(hipMemcpyAsync(h->d_args, h->h_args.data(), 16 * sizeof(StepArgs), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->d_args, h->h_args.data(), STEP_F_COUNT * sizeof(StepArgs), hipMemcpyHostToDevice,
+                        h->stream));
   return 0;
 }
 
+int launch_one(aslhip_lattice *h, unsigned fl, std::vector<hipEvent_t> *ev, int s) {
+  StepLaunch a;
+  a.nblocks = h->h_args[0].nblocks;
+  a.args = h->d_args + fl;
+  if (ev) HIPCHK(hipEventRecord((*ev)[2 * s], h->stream));
+  HIPCHK(launch_step(h->Q, h->dtype, h->next_odd, h->periodic, is_wide(h), a, h->stream));
+  if (ev) HIPCHK(hipEventRecord((*ev)[2 * s + 1], h->stream));
+  const bool was_odd = h->next_odd;
+  h->next_odd = !h->next_odd;
+  return run_vlinks(h, was_odd);
+}
+
+// Captures ASLHIP_GRAPH_CHUNK plain steps starting at parity odd.
+int build_graph(aslhip_lattice *h, bool odd) {
+  hipGraphExec_t &g = h->graph[odd ? 1 : 0];
+  if (g) return 0;
+  const bool keep = h->next_odd;
+  h->next_odd = odd;
+  const unsigned fl = base_flags(h);
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int r = 0;
+  for (int s = 0; s < ASLHIP_GRAPH_CHUNK && !r; ++s) r = launch_one(h, fl, nullptr, s);
+  hipError_t e = hipStreamEndCapture(h->stream, &graph);
+  h->next_odd = keep;
+  if (r) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r;
+  }
+  if (e != hipSuccess) return fail(ASLHIP_EHIP, "ASL ERROR: graph capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(ASLHIP_EHIP, "ASL ERROR: graph instantiate: %s", hipGetErrorString(e));
+  return 0;
+}
+
+// graph: run whole chunks of plain steps (not the raw first one, not the
+// one writing rho/u) as hipGraph replays.
 int run_steps(aslhip_lattice *h, int nsteps, bool first_after_init, bool raw_first, bool macro_last,
-              std::vector<hipEvent_t> *ev) {
+              std::vector<hipEvent_t> *ev, bool graph = false) {
   if (nsteps <= 0) return 0;
   if (int r = prepare(h, raw_first && nsteps == 1)) return r;
   if (first_after_init && !raw_first) {
@@ -604,24 +667,22 @@ int run_steps(aslhip_lattice *h, int nsteps, bool first_after_init, bool raw_fir
     if (r) return r;
   }
   if (int r = upload_args(h)) return r;
-  StepLaunch a;
-  a.nblocks = h->h_args[0].nblocks;
-  const bool wide = is_wide(h);
-  for (int s = 0; s < nsteps; ++s) {
-    unsigned fl = h->compressible ? STEP_F_COMPRESSIBLE : 0u;
-    if (s == 0 && raw_first) fl |= STEP_F_RAW;
+  for (int s = 0; s < nsteps;) {
+    const bool raw = s == 0 && raw_first;
+    if (graph && !ev && !raw && nsteps - s - (macro_last ? 1 : 0) >= ASLHIP_GRAPH_CHUNK) {
+      if (int r = build_graph(h, h->next_odd)) return r;
+      HIPCHK(hipGraphLaunch(h->graph[h->next_odd ? 1 : 0], h->stream));
+      s += ASLHIP_GRAPH_CHUNK;  // an even count: the parity is unchanged
+      continue;
+    }
+    unsigned fl = base_flags(h);
+    if (raw) fl |= STEP_F_RAW;
     if (s == nsteps - 1 && macro_last) {
       if (h->comp_rho) fl |= STEP_F_RHO;
       if (h->comp_vel) fl |= STEP_F_VEL;
     }
-    a.args = h->d_args + fl;
-    if (ev) HIPCHK(hipEventRecord((*ev)[2 * s], h->stream));
-    HIPCHK(launch_step(h->Q, h->dtype, h->next_odd, h->periodic, wide, a, h->stream));
-    if (ev) HIPCHK(hipEventRecord((*ev)[2 * s + 1], h->stream));
-    const bool was_odd = h->next_odd;
-    h->next_odd = !h->next_odd;
-    int r = run_vlinks(h, was_odd);
-    if (r) return r;
+    if (int r = launch_one(h, fl, ev, s)) return r;
+    ++s;
   }
   return 0;
 }
@@ -645,15 +706,16 @@ int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEven
   const bool first = h->fresh && nsteps > 0;
   const bool raw = first && !h->bc_since_init;
   if (nsteps > 0) h->fresh = false;
+  const bool graph = (flags & ASLHIP_STEP_GRAPH) != 0;
   if ((flags & ASLHIP_STEP_DEFER) && nsteps > 0) {
-    int r = run_steps(h, nsteps - 1, first, raw, false, ev);
+    int r = run_steps(h, nsteps - 1, first, raw, false, ev, graph);
     if (r) return r;
     h->deferred = true;
     h->deferred_first = first && nsteps == 1;
     h->deferred_raw = raw && nsteps == 1;
     return 0;
   }
-  return run_steps(h, nsteps, first, raw, (flags & ASLHIP_STEP_WRITE_MACRO) != 0, ev);
+  return run_steps(h, nsteps, first, raw, (flags & ASLHIP_STEP_WRITE_MACRO) != 0, ev, graph);
 }
 
 int copy_out(aslhip_lattice *h, const void *dev, void *host, size_t bytes) {
@@ -732,7 +794,7 @@ int aslhip_lattice_create(aslhip_lattice **out, int nx, int ny, int nz, int q, i
   if (e == hipSuccess) e = hipMalloc(&h->mode, 256);
   if (e == hipSuccess) e = hipMalloc(&h->val, 256 * (size_t)q * h->celem);
   if (e == hipSuccess) e = hipMalloc(&h->vlbc, 256 * sizeof(VLinkBC));
-  if (e == hipSuccess) e = hipMalloc(&h->d_args, 16 * sizeof(StepArgs));
+  if (e == hipSuccess) e = hipMalloc(&h->d_args, STEP_F_COUNT * sizeof(StepArgs));
   if (e == hipSuccess) e = hipMalloc(&h->red_partial, (size_t)reduce_blocks((int)len) * 5 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->red_out, 5 * sizeof(double));
   if (e != hipSuccess) {
@@ -756,14 +818,26 @@ int aslhip_lattice_destroy(aslhip_lattice *h) {
 int aslhip_set_viscosity(aslhip_lattice *h, double nu) {
   if (int r = check(h)) return r;
   if (!(nu >= 0.) && !std::isinf(nu)) return fail(ASLHIP_EINVAL, "ASL ERROR: viscosity must be >= 0");
+  if (int r = flush_deferred(h)) return r;  // a pending step runs with the old value
   h->nu = nu;
   return 0;
 }
 
 int aslhip_set_compressible(aslhip_lattice *h, int c) {
   if (int r = check(h)) return r;
+  if (int r = flush_deferred(h)) return r;
   h->compressible = c != 0;
   h->dirty = true;  // pressure-velocity map values depend on it
+  return 0;
+}
+
+int aslhip_set_omega(aslhip_lattice *h, const double omega[3]) {
+  if (int r = check(h)) return r;
+  if (omega && !(std::isfinite(omega[0]) && std::isfinite(omega[1]) && std::isfinite(omega[2])))
+    return fail(ASLHIP_EINVAL, "ASL ERROR: setOmega: non-finite angular velocity");
+  if (int r = flush_deferred(h)) return r;  // the pending step runs with the old frame
+  h->has_omega = omega != nullptr;
+  for (int d = 0; d < 3; ++d) h->omega[d] = omega ? omega[d] : 0.;
   return 0;
 }
 
@@ -996,13 +1070,15 @@ int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags, double *ms_
     int r = aslhip_finalize(h);
     if (r) return r;
   }
-  std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
+  // replayed graphs have no events between their launches
+  const bool graph = (flags & ASLHIP_STEP_GRAPH) != 0;
+  std::vector<hipEvent_t> ev(graph ? 0 : 2 * (size_t)nsteps);
   for (auto &e : ev) HIPCHK(hipEventCreate(&e));
   hipEvent_t t0, t1;
   HIPCHK(hipEventCreate(&t0));
   HIPCHK(hipEventCreate(&t1));
   HIPCHK(hipEventRecord(t0, h->stream));
-  int r = step_impl(h, nsteps, flags, &ev);
+  int r = step_impl(h, nsteps, flags, graph ? nullptr : &ev);
   HIPCHK(hipEventRecord(t1, h->stream));
   HIPCHK(hipEventSynchronize(t1));
   if (r == 0) {
@@ -1010,7 +1086,7 @@ int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags, double *ms_
     HIPCHK(hipEventElapsedTime(&ms, t0, t1));
     if (ms_total) *ms_total = ms;
     double ksum = 0.;
-    for (int s = 0; s < nsteps; ++s) {
+    for (int s = 0; s < (graph ? 0 : nsteps); ++s) {
       float k = 0.f;
       HIPCHK(hipEventElapsedTime(&k, ev[2 * s], ev[2 * s + 1]));
       ksum += k;

@@ -67,6 +67,13 @@ class Lattice:
     def set_compressible(self, flag=True):
         call("aslhip_set_compressible", self.h, 1 if flag else 0)
 
+    def set_omega(self, omega):
+        """LBGK::setOmega: angular velocity (3 components) of the rotating frame; None turns it off."""
+        if omega is None:
+            call("aslhip_set_omega", self.h, None)
+        else:
+            call("aslhip_set_omega", self.h, (ctypes.c_double * 3)(*[float(x) for x in omega]))
+
     def set_compute_macro(self, rho=True, vel=True):
         call("aslhip_set_compute_macro", self.h, int(bool(rho)), int(bool(vel)))
 
@@ -108,8 +115,9 @@ class Lattice:
         call("aslhip_finalize", self.h)
 
     # stepping -------------------------------------------------------------
-    def step(self, nsteps=1, write_macro=True, defer=False):
-        flags = (STEP_WRITE_MACRO if write_macro else 0) | (STEP_DEFER if defer else 0)
+    def step(self, nsteps=1, write_macro=True, defer=False, graph=False):
+        flags = ((STEP_WRITE_MACRO if write_macro else 0) | (STEP_DEFER if defer else 0)
+                 | (STEP_GRAPH if graph else 0))
         call("aslhip_step", self.h, int(nsteps), flags)
 
     def synchronize(self):
@@ -118,11 +126,12 @@ class Lattice:
     def set_tuning(self, knob, value):
         call("aslhip_set_tuning", self.h, int(knob), int(value))
 
-    def time_steps(self, nsteps, write_macro=False):
+    def time_steps(self, nsteps, write_macro=False, graph=False):
+        """(ms of the whole call, summed ms of the step kernels; 0 with graph=True)."""
         tot = ctypes.c_double(0)
         ker = ctypes.c_double(0)
-        call("aslhip_time_steps", self.h, int(nsteps), STEP_WRITE_MACRO if write_macro else 0, ctypes.byref(tot),
-             ctypes.byref(ker))
+        flags = (STEP_WRITE_MACRO if write_macro else 0) | (STEP_GRAPH if graph else 0)
+        call("aslhip_time_steps", self.h, int(nsteps), flags, ctypes.byref(tot), ctypes.byref(ker))
         return tot.value, ker.value
 
     # host views -----------------------------------------------------------

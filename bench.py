@@ -18,6 +18,8 @@ launch duration, measured with HIP events recorded on the lattice's stream.
 a bounded sample of the same lattice with the host's cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       python bench.py --copy-calibration BYTES   (one plain device copy, for
+       scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters)
 """
 import argparse
 import json
@@ -39,15 +41,37 @@ U_IN = 0.04
 SEED = 20261202
 
 
+def splitmix64_uniform(seed, n):
+    """Outputs 0..n-1 of the splitmix64 generator seeded with seed, as floats in [-1, 1).
+
+    splitmix64 (S. Vigna, public domain): output k is mix(seed + (k+1) *
+    0x9E3779B97F4A7C15).  The top 24 bits give an exact float32 in [0, 1),
+    mapped to 2u - 1.  Counter-based, so any host can rebuild the field.
+    """
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (np.arange(n, dtype=np.uint64) + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z ^= z >> np.uint64(31)
+    u01 = (z >> np.uint64(40)).astype(np.float32) * np.float32(2.0 ** -24)
+    return np.float32(2.0) * u01 - np.float32(1.0)
+
+
 def setup_fields():
-    """Sphere map and the perturbed initial velocity (float32, ASL layout)."""
+    """Sphere map and the perturbed initial velocity (float32, ASL layout).
+
+    u = (U_IN, 0, 0) + 1e-3 * xi, xi in [-1, 1) from splitmix64 output
+    3 * cell + component (cell: ghost-inclusive linear index), u = 0 where
+    the map is <= 0 (inside the sphere).
+    """
     from asl_amd import geometry
     mapv = geometry.normalize(geometry.sphere(SIZE, 16.0, (64.0, 64.0, 64.0)))
     S = tuple(s + 2 for s in SIZE)
-    rng = np.random.default_rng(SEED)
+    ncell = int(np.prod(S))
+    xi = splitmix64_uniform(SEED, 3 * ncell).reshape(S + (3,))
     u = np.zeros((3,) + S, dtype=np.float32)
     u[0] = U_IN
-    u += np.float32(1e-3) * rng.uniform(-1.0, 1.0, size=u.shape).astype(np.float32)
+    u += np.float32(1e-3) * np.moveaxis(xi, -1, 0)
     u[:, mapv <= 0] = 0.0
     return mapv, u
 
@@ -94,6 +118,16 @@ def cpu_baseline(mapv, u, budget_s=15.0):
                       "sphere lattice, %d steps in %.1f s" % (steps, el)}
 
 
+def copy_calibration(nbytes):
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("copied %d bytes device to device" % (a.numel() * 4), flush=True)
+
+
 def profile_traffic():
     """HBM bytes per launch of the step kernel from the committed rocprofv3 summary, or None."""
     path = os.path.join(ROOT, "profiles", "lbgk_step_d3q19_f32_sphere_256x128x128_traffic.md")
@@ -109,7 +143,13 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline sample")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES bytes device to device once and exit (reference kernel of known "
+                         "traffic for the HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this LBGK build runs on one GPU only; --gpus %d is not supported "
               "(the multi-GPU slab decomposition is a design, not built)" % args.gpus, file=sys.stderr)

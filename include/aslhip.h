@@ -80,6 +80,9 @@ extern "C" {
 #define ASLHIP_STEP_DEFER 2u       /* keep the last step pending: it runs, writing
                                       rho and u, when the host next reads or
                                       synchronises, or before the next step */
+#define ASLHIP_STEP_GRAPH 4u       /* run whole chunks of ASLHIP_GRAPH_CHUNK
+                                      steps as replays of captured hipGraphs */
+#define ASLHIP_GRAPH_CHUNK 100
 
 typedef struct aslhip_lattice aslhip_lattice;
 
@@ -110,6 +113,13 @@ int aslhip_set_viscosity(aslhip_lattice *h, double nu);
 /* asl::LBGK::setCompressible — src/num/aslLBGK.h:129, equilibrium variants
  * at src/num/aslLBGK.cxx:220-231. */
 int aslhip_set_compressible(aslhip_lattice *h, int compressible);
+
+/* asl::LBGK::setOmega(Param w) — src/num/aslLBGK.h:120,
+ * src/num/aslLBGK.cxx:207-208,252-255.  Rotating frame with angular velocity
+ * omega[3] (lattice units): each collision first sets
+ * momentum += (2/w) omega x momentum, component by component.  NULL turns
+ * it off. */
+int aslhip_set_omega(aslhip_lattice *h, const double omega[3]);
 
 /* compRho / compVel constructor flags (src/num/aslLBGK.cxx:210-218). */
 int aslhip_set_compute_macro(aslhip_lattice *h, int comp_rho, int comp_vel);
@@ -192,7 +202,8 @@ int aslhip_set_tuning(aslhip_lattice *h, int knob, int value);
 
 /* Times nsteps steps with HIP events recorded on the lattice's stream.
  * ms_total: wall time of the whole call between the events; ms_kernel:
- * summed time of the step kernels (events around each launch). */
+ * summed time of the step kernels (events around each launch; 0 with
+ * ASLHIP_STEP_GRAPH, whose replays carry no events between launches). */
 int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags,
                       double *ms_total, double *ms_kernel);
 
@@ -203,8 +214,6 @@ int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags,
  * All lattices of a batch share the template, population type and
  * periodicity, and have no velocity-map BCs.  ASLHIP_STEP_GRAPH replays
  * hipGraph-captured chunks of ASLHIP_GRAPH_CHUNK steps. */
-#define ASLHIP_STEP_GRAPH 4u
-#define ASLHIP_GRAPH_CHUNK 100
 typedef struct aslhip_batch aslhip_batch;
 int aslhip_batch_create(aslhip_batch **out, aslhip_lattice *const *lattices, int n);
 int aslhip_batch_destroy(aslhip_batch *b);

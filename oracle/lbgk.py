@@ -33,6 +33,7 @@ def load_library():
     lib.orc_set_threads.restype = ctypes.c_int
     lib.orc_set_viscosity.argtypes = [ctypes.c_void_p, ctypes.c_double]
     lib.orc_set_compressible.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_set_omega.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     lib.orc_set_compute_macro.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     lib.orc_set_geometry.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
     lib.orc_init_equilibrium.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
@@ -110,6 +111,14 @@ class OracleLattice:
 
     def set_compressible(self, flag=True):
         self.lib.orc_set_compressible(self.h, 1 if flag else 0)
+
+    def set_omega(self, omega):
+        """LBGK::setOmega: angular velocity (3 floats) of the rotating frame, None: off."""
+        if omega is None:
+            self.lib.orc_set_omega(self.h, None)
+        else:
+            self._omega = np.ascontiguousarray(np.asarray(omega, dtype=np.float64).reshape(3))
+            self.lib.orc_set_omega(self.h, self._omega.ctypes.data)
 
     def set_compute_macro(self, rho=True, vel=True):
         self.lib.orc_set_compute_macro(self.h, int(rho), int(vel))

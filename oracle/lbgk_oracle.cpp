@@ -22,6 +22,7 @@
 #include <limits>
 #include <memory>
 #include <stdexcept>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -139,6 +140,10 @@ struct Lattice {
   std::vector<T> rho, vel;
   T nu = T(0.1);
   bool compressible = true, comp_rho = true, comp_vel = true;
+  // LBGK::setOmega (src/num/aslLBGK.cxx:252-255): angular velocity of the
+  // rotating frame, constants rounded to T like ASL's literals
+  bool has_omega = false;
+  T om[3] = {0, 0, 0};
   double dx = 1., pos0[3] = {0., 0., 0.};
   std::vector<BC<T>> bcs;
   // per-direction constants as ASL prints them (double -> T)
@@ -219,6 +224,17 @@ struct Lattice {
     }
     T r = f[0] * cw[0];
     for (int j = 1; j < Q; ++j) r = std::fma(f[j], cw[j], r);
+    // momentum += (2./w) * crossProduct(omega, momentum) (:207-208).  The
+    // three components are three statements of the generated kernel, in
+    // order (operator+=, src/acl/aclMath/aclVectorOfElementsOperations.cxx:249-259;
+    // ExpressionContainer <<, src/acl/acl.cxx:578-584), so each one sees the
+    // components updated before it.  crossProduct: :514-535.
+    if (has_omega) {
+      const T t = T(2.) / w;
+      m[0] = std::fma(t, std::fma(om[1], m[2], -(om[2] * m[1])), m[0]);
+      m[1] = std::fma(t, std::fma(om[2], m[0], -(om[0] * m[2])), m[1]);
+      m[2] = std::fma(t, std::fma(om[0], m[1], -(om[1] * m[0])), m[2]);
+    }
     rho_o = r;
     if (compressible)
       for (int d = 0; d < 3; ++d) v_o[d] = m[d] / r;
@@ -561,6 +577,13 @@ void orc_set_viscosity(void *h, double nu) {
 }
 void orc_set_compressible(void *h, int c) {
   dispatch(h, [&](auto &L) { L.compressible = c != 0; });
+}
+void orc_set_omega(void *h, const double *om) {
+  dispatch(h, [&](auto &L) {
+    using T = std::remove_reference_t<decltype(L.om[0])>;
+    L.has_omega = om != nullptr;
+    for (int d = 0; d < 3; ++d) L.om[d] = om ? T(om[d]) : T(0);
+  });
 }
 void orc_set_compute_macro(void *h, int r, int v) {
   dispatch(h, [&](auto &L) {

@@ -35,6 +35,7 @@ void orc_destroy(void *h);
 int orc_set_threads(int n);
 void orc_set_viscosity(void *h, double nu);
 void orc_set_compressible(void *h, int c);
+void orc_set_omega(void *h, const double *omega); /* NULL: off */
 void orc_set_compute_macro(void *h, int comp_rho, int comp_vel);
 void orc_set_geometry(void *h, double dx, const double pos[3]);
 void orc_init_equilibrium(void *h, double rho, const double u[3]);

@@ -202,3 +202,29 @@ def test_asl_style_driver_with_deferred_steps():
         s = (slice(1, -1),) * 3
         assert np.array_equal(lb.getRho()[s], O.rho()[s].astype(np.float32)), pre
         assert np.array_equal(lb.getVelocity()[(slice(None),) + s], O.velocity()[(slice(None),) + s].astype(np.float32))
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+@pytest.mark.parametrize("compressible", [True, False])
+def test_rotating_frame(dtype, compressible):
+    """LBGK::setOmega: cavity in a rotating frame, GPU vs oracle bit for bit;
+    the term switched on after some plain steps (and off again) as a driver may."""
+    G, O = pair((14, 12, 10), 19, nu=0.04, dtype=dtype, compressible=compressible)
+    G.init_equilibrium(1.0, (0.01, 0, 0))
+    O.init_f(1.0, (0.01, 0, 0))
+    bcs = [add_bc(G, O, _lib.BC_CONSTANT_VELOCITY, _lib.ZE, u=(0.05, 0, 0)),
+           add_bc(G, O, _lib.BC_NOSLIP, ALL_BUT_ZE)]
+    run(G, O, 20, bcs)
+    om = (0.004, -0.002, 0.01)
+    G.set_omega(om)
+    O.set_omega(om)
+    G.step(150)
+    O.run(150, [o for _, o in bcs])
+    d = compare(G, O)
+    assert d["rho"] == 0.0 and d["u"] == 0.0 and d["f"] == 0.0, d
+    G.set_omega(None)
+    O.set_omega(None)
+    G.step(30)
+    O.run(30, [o for _, o in bcs])
+    d = compare(G, O)
+    assert d["rho"] == 0.0 and d["u"] == 0.0 and d["f"] == 0.0, d

@@ -154,3 +154,78 @@ def test_taylor_green_decay(q):
     assert ke1 / ke0 == pytest.approx(expect, rel=0.02)
     # a wrong viscosity relation (e.g. nu = (tau - 1/2)/6) would miss by far more
     assert abs(ke1 / ke0 - np.exp(-2 * nu * k * k * steps)) > 0.1
+
+
+def rotate_sequential(m, om, t):
+    """ASL's momentum += t * (omega x momentum), one component after the other."""
+    m = m.copy()
+    m[0] = m[0] + t * (om[1] * m[2] - om[2] * m[1])
+    m[1] = m[1] + t * (om[2] * m[0] - om[0] * m[2])
+    m[2] = m[2] + t * (om[0] * m[1] - om[1] * m[0])
+    return m
+
+
+@pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("compressible", [True, False])
+def test_rotating_frame_one_step(q, compressible):
+    """setOmega: textbook BGK with the momentum first moved by (2/w) omega x m.
+
+    The expected value is built in numpy from the moments of f; the
+    component-by-component update (ASL emits one statement per component) is
+    distinguishable from the simultaneous one at these sizes, and so are a
+    wrong sign, factor or operand order of the cross product.
+    """
+    size, nu, om = (5, 6, 4), 0.07, np.array([0.02, -0.03, 0.05])
+    c, w, _ = template(q)
+    rng = np.random.default_rng(5)
+    L = OracleLattice(size, q, fp64=True, nu=nu, periodic=True)
+    L.set_compressible(compressible)
+    L.set_omega(om)
+    S = tuple(s + 2 for s in size)
+    f0 = np.full((q,) + S, np.nan)
+    f0[:, 1:-1, 1:-1, 1:-1] = 1.0 + 0.05 * rng.standard_normal((q,) + size)
+    L.write_f(f0)
+    L.execute()
+    f = pull_periodic(f0[:, 1:-1, 1:-1, 1:-1], c)
+    rho = np.einsum("i,i...->...", w, f)
+    m = np.einsum("i,ia,i...->a...", w, c, f)
+    wr = 1.0 / (3.0 * nu + 0.5)
+    m2 = rotate_sequential(m, om, 2.0 / wr)
+    ref = np.empty_like(f)
+    for i in range(q):
+        cm = np.einsum("a,a...->...", c[i], m2)
+        if compressible:
+            feq = rho + 3 * cm + 4.5 * cm * cm / rho - 1.5 * (m2 * m2).sum(0) / rho
+        else:
+            feq = rho + 3 * cm + 4.5 * cm * cm - 1.5 * (m2 * m2).sum(0)
+        ref[i] = f[i] - wr * (f[i] - feq)
+    got = L.f()[:, 1:-1, 1:-1, 1:-1]
+    assert np.abs(got - ref).max() < 4e-15
+    v = L.velocity()[:, 1:-1, 1:-1, 1:-1]
+    assert np.abs(v - (m2 / rho if compressible else m2)).max() < 4e-15
+    # the simultaneous update is a different (wrong) answer at this size
+    m_sim = m + (2.0 / wr) * np.cross(om, m, axis=0)
+    assert np.abs(m_sim - m2).max() > 1e-6
+    # momentum after the collision: m + w (m' - m), i.e. m + 2 omega x m to first order
+    mom = np.einsum("i,ia,i...->a...", w, c, got)
+    assert np.abs(mom - (m + wr * (m2 - m))).max() < 4e-15
+
+
+def test_rotating_frame_off_is_plain_bgk():
+    size = (4, 4, 4)
+    rng = np.random.default_rng(3)
+    S = tuple(s + 2 for s in size)
+    f0 = np.full((19,) + S, np.nan)
+    f0[:, 1:-1, 1:-1, 1:-1] = 1.0 + 0.05 * rng.standard_normal((19,) + size)
+    out = []
+    for om in (None, (0.0, 0.0, 0.0), "off"):
+        L = OracleLattice(size, 19, fp64=True, nu=0.1, periodic=True)
+        if om == "off":
+            L.set_omega((0.1, 0.2, 0.3))
+            L.set_omega(None)
+        elif om is not None:
+            L.set_omega(om)
+        L.write_f(f0)
+        L.execute()
+        out.append(L.f()[:, 1:-1, 1:-1, 1:-1])
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
