@@ -20,6 +20,7 @@ STEP_GRAPH = 4
 GRAPH_CHUNK = 100
 CODE_FLUID, CODE_FLUID_BC, CODE_SKIP, CODE_GHOST = 0, 1, 2, 3
 TUNE_XCD_REMAP = 1
+CLOSURE_NPARAM = 1345
 
 # Every entry point of include/aslhip.h with its ctypes signature.
 _vp = ctypes.c_void_p
@@ -56,6 +57,14 @@ SIGNATURES = {
     "aslhip_batch_destroy": [_vp],
     "aslhip_batch_step": [_vp, ctypes.c_int, ctypes.c_uint],
     "aslhip_batch_time_steps": [_vp, ctypes.c_int, ctypes.c_uint, _dp],
+    "aslhip_closure_set": [_vp, _fp, _fp, _fp, ctypes.c_float],
+    "aslhip_closure_features": [_vp, _fp],
+    "aslhip_mlp_create": [ctypes.POINTER(_vp), _fp],
+    "aslhip_mlp_destroy": [_vp],
+    "aslhip_mlp_set_data": [_vp, _fp, _fp, ctypes.c_longlong],
+    "aslhip_mlp_gradient": [_vp, ctypes.c_longlong, ctypes.c_int, _fp, _dp],
+    "aslhip_mlp_train": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _dp],
+    "aslhip_mlp_get_params": [_vp, _fp],
 }
 
 _LIB = None

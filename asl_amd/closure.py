@@ -1,0 +1,184 @@
+"""Learned closure for coarse lattices: data, training and evaluation (host side).
+
+A coarse D3Q19 lattice (4x coarser than a reference run) gets a per-cell
+relaxation-time correction from a small network of its own non-equilibrium
+moments (the network and its inputs: ``aslhip_closure_set`` in
+``include/aslhip.h``).  It is trained on box-filtered runs of a 4x finer
+lattice produced by this same library:
+
+* the fine run starts from a seeded random-phase solenoidal field with
+  energy spectrum E(k) ~ k^4 exp(-2 (k/4)^2) scaled to u_rms;
+* at each snapshot the fine velocity is box-filtered by 4; the subgrid
+  stress tau_ij = <u_i u_j> - <u_i><u_j> is projected on the filtered strain
+  rate S_ij (central differences) to an eddy viscosity
+  nu_t = max(0, -<tau^d : S>_3 / (2 <S : S>_3)) with <.>_3 a 3^3 box average
+  (the least-squares estimate of the dynamic Smagorinsky procedure), and the
+  target relaxation-time correction is 3 nu_t;
+* the coarse lattice, started from the filtered fields and run two steps,
+  gives the network inputs of every cell.
+
+The network trains on the GPU with the library's own forward / backward /
+Adam kernels (``asl_amd.Mlp``).  Plain LBGK stays the fallback
+(``set_closure(None)``) and the judge of the error.  Acoustic scaling: one
+coarse step is four fine steps at the same lattice velocity, and the coarse
+viscosity is a quarter of the fine one.
+"""
+import time
+
+import numpy as np
+
+from .lattice import Lattice, Mlp
+
+
+def turbulence_field(n, seed, u_rms=0.03, kpeak=4.0):
+    """Random-phase solenoidal velocity on a periodic n^3 grid (float32, shape (3, n, n, n)).
+
+    White noise filtered in Fourier space by k exp(-(k/kpeak)^2) (so the shell
+    spectrum is E(k) ~ k^4 exp(-2 (k/kpeak)^2)), projected onto divergence-free
+    modes, mean removed, scaled to u_rms = sqrt(<|u|^2> / 3).
+    """
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, d=1.0 / n).astype(np.float32)
+    kz = np.fft.rfftfreq(n, d=1.0 / n).astype(np.float32)
+    KX, KY, KZ = np.meshgrid(k1, k1, kz, indexing="ij")
+    k2 = KX * KX + KY * KY + KZ * KZ
+    kk = np.sqrt(k2)
+    amp = kk * np.exp(-(kk / np.float32(kpeak)) ** 2)
+    amp[0, 0, 0] = 0.0
+    uh = []
+    for _ in range(3):
+        w = rng.standard_normal((n, n, n), dtype=np.float32)
+        uh.append(np.fft.rfftn(w) * amp)
+    k2[0, 0, 0] = 1.0
+    div = (KX * uh[0] + KY * uh[1] + KZ * uh[2]) / k2
+    uh[0] -= KX * div
+    uh[1] -= KY * div
+    uh[2] -= KZ * div
+    del div, KX, KY, KZ, k2, kk, amp
+    u = np.stack([np.fft.irfftn(c, s=(n, n, n)).astype(np.float32) for c in uh])
+    del uh
+    u -= u.mean(axis=(1, 2, 3), keepdims=True)
+    u *= np.float32(u_rms / np.sqrt((u * u).sum(0).mean() / 3.0))
+    return u
+
+
+def with_ghosts(a):
+    """Periodic ghost layer around the last three axes (ASL's ghost-inclusive layout)."""
+    return np.pad(a, [(0, 0)] * (a.ndim - 3) + [(1, 1)] * 3, mode="wrap")
+
+
+def box_filter(a, r=4):
+    """Mean over r^3 blocks of the last three axes."""
+    s = a.shape[-3:]
+    b = a.reshape(a.shape[:-3] + (s[0] // r, r, s[1] // r, r, s[2] // r, r))
+    nd = a.ndim - 3
+    return b.mean(axis=(nd + 1, nd + 3, nd + 5))
+
+
+def box3(a):
+    """Periodic 3^3 box average over the last three axes."""
+    out = np.zeros_like(a)
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                out += np.roll(a, (dx, dy, dz), axis=(-3, -2, -1))
+    return out / 27.0
+
+
+def strain_rate(u):
+    """S_ij = (d_i u_j + d_j u_i) / 2 by periodic central differences, as (xx, yy, zz, xy, xz, yz)."""
+    d = [[(np.roll(u[j], -1, axis=i) - np.roll(u[j], 1, axis=i)) * 0.5 for j in range(3)] for i in range(3)]
+    return np.stack([d[0][0], d[1][1], d[2][2], 0.5 * (d[0][1] + d[1][0]), 0.5 * (d[0][2] + d[2][0]),
+                     0.5 * (d[1][2] + d[2][1])])
+
+
+def eddy_viscosity(u_fine, r=4):
+    """Least-squares eddy viscosity of the box-filtered field (coarse lattice units)."""
+    ub = box_filter(u_fine, r)
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    tau = np.stack([box_filter(u_fine[i] * u_fine[j], r) - ub[i] * ub[j] for i, j in pairs])
+    tr = (tau[0] + tau[1] + tau[2]) / 3.0
+    tau[:3] -= tr
+    S = strain_rate(ub)
+    wts = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0]).reshape(6, 1, 1, 1)  # off-diagonal pairs count twice
+    num = box3((wts * tau * S).sum(0))
+    den = box3((wts * S * S).sum(0))
+    return np.maximum(0.0, -num / (2.0 * np.maximum(den, 1e-30))), ub
+
+
+class Experiment:
+    """Fine / coarse decaying-turbulence runs sharing one initial field."""
+
+    def __init__(self, n_coarse=128, tau_fine=0.55, u_rms=0.03, seed=20261202, r=4):
+        self.nc, self.r = n_coarse, r
+        self.nf = n_coarse * r
+        self.nu_f = (tau_fine - 0.5) / 3.0
+        self.nu_c = self.nu_f / r
+        self.seed, self.u_rms = seed, u_rms
+        self.u0 = turbulence_field(self.nf, seed, u_rms)
+
+    def coarse(self, u=None, rho=None):
+        L = Lattice((self.nc,) * 3, 19, dtype="f32", periodic=True, nu=self.nu_c)
+        ub = box_filter(self.u0) if u is None else u
+        L.init_equilibrium(1.0 if rho is None else with_ghosts(rho), with_ghosts(ub))
+        return L
+
+    def fine(self):
+        L = Lattice((self.nf,) * 3, 19, dtype="f32", periodic=True, nu=self.nu_f)
+        L.init_equilibrium(1.0, with_ghosts(self.u0))
+        return L
+
+    def training_data(self, coarse_steps_between, snapshots, log=print):
+        """(features n x 7, targets n) from the fine run at `snapshots` coarse-step times."""
+        F = self.fine()
+        inner = (slice(None),) + (slice(1, -1),) * 3
+        xs, ts, done = [], [], 0
+        for snap in range(1, snapshots + 1):
+            F.step((snap * coarse_steps_between - done) * self.r)
+            done = snap * coarse_steps_between
+            u = F.velocity()[inner]
+            rho = box_filter(F.rho()[inner[1:]])
+            nut, ub = eddy_viscosity(u, self.r)
+            del u
+            C = self.coarse(ub.astype(np.float32), rho.astype(np.float32))
+            C.step(2)
+            xs.append(C.closure_features().reshape(-1, 7))
+            ts.append((3.0 * nut).astype(np.float32).reshape(-1))
+            log("# closure data: snapshot %d (coarse step %d): mean target %.3e" % (snap, done, ts[-1].mean()))
+        del F
+        return np.concatenate(xs), np.concatenate(ts)
+
+    def ke_history(self, lattice, steps_between, count, fine=False):
+        """Mean kinetic energy per cell after each of `count` chunks (coarse-step units)."""
+        mult = self.r if fine else 1
+        ncell = float((self.nf if fine else self.nc) ** 3)
+        out = [lattice.conservation()[4] / ncell]
+        t0 = time.perf_counter()
+        for _ in range(count):
+            lattice.step(steps_between * mult, write_macro=False, graph=True)
+            out.append(lattice.conservation()[4] / ncell)
+        lattice.synchronize()
+        return np.array(out), time.perf_counter() - t0
+
+
+def normalisation(x, t):
+    """Input shift / scale and output scale of a training set."""
+    mu = x.mean(0).astype(np.float32)
+    sd = x.std(0)
+    sc = (1.0 / np.where(sd > 0, sd, 1.0)).astype(np.float32)
+    out = float(t.std()) if t.std() > 0 else 1.0
+    return mu, sc, out
+
+
+def train(x, t, params0, steps=3000, batch=8192, lr=2e-3, seed=1):
+    """Train on the GPU; returns (params, mu, sc, out_scale, losses)."""
+    mu, sc, out = normalisation(x, t)
+    xn = ((x - mu) * sc).astype(np.float32)
+    m = Mlp(params0)
+    m.set_data(xn, (t / out).astype(np.float32))
+    losses = m.train(steps, batch, lr, seed)
+    return m.params(), mu, sc, out, losses
+
+
+__all__ = ["Experiment", "turbulence_field", "box_filter", "eddy_viscosity", "strain_rate", "train",
+           "normalisation", "with_ghosts"]

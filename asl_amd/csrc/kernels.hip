@@ -113,18 +113,11 @@ __device__ __forceinline__ C dot_c(const C *u, int i) {
   return a;
 }
 
-// The BGK collision of one cell: ASL's LBGK::init0 (src/num/aslLBGK.cxx:
-// 187-235) with computeRho / computeMomentum / a_i_dot_v (:46-72).  Each
-// x*y + z of ASL's expression tree is one fma (OpenCL's default FP_CONTRACT
-// ON); the CPU oracle contracts the same sites.  Opposite directions share
-// (4.5 a^2)/rho: a_inv(i) = -a_i exactly in IEEE arithmetic, so this is the
-// same value ASL computes twice.  On return v holds the post-collision
-// populations, r the density and m the momentum.
+// computeMomentum / computeRho (src/num/aslLBGK.cxx:46-63): m = sum w_i c_i f_i
+// and rho = sum w_i f_i, left-to-right fma chains without the zero terms.
 template <int Q, class C>
-__device__ __forceinline__ void collide(C (&v)[Q], C w, bool compressible, bool omega, const C (&om)[3], C &r_o,
-                                        C (&m_o)[3]) {
+__device__ __forceinline__ void moments(const C (&v)[Q], C &r, C (&m)[3]) {
   using LT = Lat<Q>;
-  C m[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     C acc = C(0);
@@ -138,9 +131,29 @@ __device__ __forceinline__ void collide(C (&v)[Q], C w, bool compressible, bool 
     }
     m[d] = acc;
   }
-  C r = v[0] * C(LT::w(0));
+  r = v[0] * C(LT::w(0));
 #pragma unroll
   for (int j = 1; j < Q; ++j) r = fmaT(v[j], C(LT::w(j)), r);
+}
+
+// The BGK collision of one cell: ASL's LBGK::init0 (src/num/aslLBGK.cxx:
+// 187-235) with computeRho / computeMomentum / a_i_dot_v (:46-72).  Each
+// x*y + z of ASL's expression tree is one fma (OpenCL's default FP_CONTRACT
+// ON); the CPU oracle contracts the same sites.  Opposite directions share
+// (4.5 a^2)/rho: a_inv(i) = -a_i exactly in IEEE arithmetic, so this is the
+// same value ASL computes twice.  On return v holds the post-collision
+// populations, r the density and m the momentum.  CL: the closure network
+// replaces w per cell (FP32 collision only).
+template <int Q, class C, bool CL = false>
+__device__ __forceinline__ void collide(C (&v)[Q], C w, bool compressible, bool omega, const C (&om)[3], C &r_o,
+                                        C (&m_o)[3], const ClosureParams *cl = nullptr, float tau0 = 0.f) {
+  C m[3], r;
+  moments<Q, C>(v, r, m);
+  if constexpr (CL) {  // the learned closure sets the cell's relaxation
+    float x[7];
+    closure_features<Q>(v, r, m, x);
+    w = closure_w(x, *cl, tau0);
+  }
   // LBGK::setOmega (src/num/aslLBGK.cxx:207-208): momentum += (2/w) omega x
   // momentum, one component after the other, each seeing the ones before it
   // (ASL emits one statement per component).
@@ -201,7 +214,7 @@ __device__ __forceinline__ double face_val<double>(const StepArgs &a, int f, int
 }
 
 // One cell of one step: chunk L of the lattice described by a.
-template <int Q, class S, bool ODD, bool PER, bool WIDE>
+template <int Q, class S, bool ODD, bool PER, bool WIDE, bool CL = false>
 __device__ __forceinline__ void step_body(const StepArgs &a, const int L) {
   using CD = Codec<S>;
   using C = typename CD::C;
@@ -336,7 +349,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) om[d] = sizeof(C) == 4 ? C(a.om32[d]) : C(a.om64[d]);
   }
-  collide<Q, C>(v, C(a.w), (a.flags & STEP_F_COMPRESSIBLE) != 0, omega, om, r, m);
+  collide<Q, C, CL>(v, C(a.w), (a.flags & STEP_F_COMPRESSIBLE) != 0, omega, om, r, m, a.cl, a.cl_tau0);
 
   if (!live) return;
   // Stores address every direction from fd[0] too (one scalar base pair
@@ -371,10 +384,10 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L) {
 // of them (27 direction pointers, face tables) into scalar registers, runs
 // out of them and spills into vector registers (D3Q19 FP32: 103 VGPRs
 // instead of 70).  Loads through a plain pointer stay where they are used.
-template <int Q, class S, bool ODD, bool PER, bool WIDE>
+template <int Q, class S, bool ODD, bool PER, bool WIDE, bool CL>
 __global__ __launch_bounds__(STEP_BLOCK) void k_step(const StepArgs *args) {
   const StepArgs &a = *args;
-  step_body<Q, S, ODD, PER, WIDE>(a, a.remap ? xcd_remap(blockIdx.x, a.nblocks) : static_cast<int>(blockIdx.x));
+  step_body<Q, S, ODD, PER, WIDE, CL>(a, a.remap ? xcd_remap(blockIdx.x, a.nblocks) : static_cast<int>(blockIdx.x));
 }
 
 // Many lattices in one launch: block b works on lattice lat[b], chunk
@@ -385,6 +398,44 @@ __global__ __launch_bounds__(STEP_BLOCK) void k_step_batch(const StepArgs *desc,
   const int l = lat[blockIdx.x];
   const StepArgs &a = desc[l];
   step_body<Q, S, ODD, PER, WIDE>(a, static_cast<int>(blockIdx.x) - first[l]);
+}
+
+// Closure features of every interior cell of a periodic D3Q19 lattice: the
+// populations the next step would collide (AA addressing of that step's
+// parity, periodic wrap), their moments as collide() forms them, and the 7
+// inputs of the closure network, written to out[7 * cell + p] with cells in
+// ASL order over the interior.
+template <class S, bool ODD>
+__global__ __launch_bounds__(256) void k_closure_features(const StepArgs *args, float *out) {
+  using CD = Codec<S>;
+  constexpr int Q = 19;
+  constexpr unsigned ES = sizeof(S);
+  const StepArgs &a = *args;
+  const long long c = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (c >= static_cast<long long>(a.nx) * a.ny * a.nz) return;
+  const int z = static_cast<int>(c % a.nz) + 1;
+  const int y = static_cast<int>((c / a.nz) % a.ny) + 1;
+  const int x = static_cast<int>(c / (static_cast<long long>(a.ny) * a.nz)) + 1;
+  const unsigned idx = static_cast<unsigned>(x * a.sx + y * a.sy + z);
+  float v[Q];
+#pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    unsigned o;
+    if (ODD) {
+      const unsigned src = static_cast<unsigned>(wrap1(x - Lat<Q>::c[k][0], a.nx) * a.sx +
+                                                 wrap1(y - Lat<Q>::c[k][1], a.ny) * a.sy +
+                                                 wrap1(z - Lat<Q>::c[k][2], a.nz));
+      o = static_cast<unsigned>(inv_dir<Q>(k)) * a.qs32 + ES * src;
+    } else {
+      o = static_cast<unsigned>(k) * a.qs32 + ES * idx;
+    }
+    v[k] = CD::ld(a.fd[0], o);
+  }
+  float r, m[3], f7[7];
+  moments<Q, float>(v, r, m);
+  closure_features<Q>(v, r, m, f7);
+#pragma unroll
+  for (int p = 0; p < 7; ++p) out[7 * c + p] = f7[p];
 }
 
 // Boundary-link kernel: ASL's BCVelocityMap (src/num/aslLBGKBC.cxx:290-348).
@@ -557,10 +608,22 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_final(const double *partial, i
 template <int Q, class S, bool ODD, bool PER>
 hipError_t step_w(bool wide, const StepLaunch &a, hipStream_t s) {
   const dim3 grid(a.nblocks), block(STEP_BLOCK);
+  if (a.closure) {
+    // the closure variants exist for D3Q19 with FP32 collision only
+    if constexpr (Q == 19 && sizeof(typename Codec<S>::C) == 4) {
+      if (wide)
+        hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true, true>), grid, block, 0, s, a.args);
+      else
+        hipLaunchKernelGGL((k_step<Q, S, ODD, PER, false, true>), grid, block, 0, s, a.args);
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
   if (wide)
-    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true>), grid, block, 0, s, a.args);
+    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true, false>), grid, block, 0, s, a.args);
   else
-    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, false>), grid, block, 0, s, a.args);
+    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, false, false>), grid, block, 0, s, a.args);
   return hipGetLastError();
 }
 
@@ -664,6 +727,25 @@ hipError_t launch_vlinks(int q, int dtype, const VLinkArgs &a, hipStream_t s) {
 hipError_t launch_init(int q, int dtype, const InitArgs &a, hipStream_t s) { return dispatch<InitF>(q, dtype, a, s); }
 hipError_t launch_unpermute(int q, int dtype, const UnpermuteArgs &a, hipStream_t s) {
   return dispatch<UnpermuteF>(q, dtype, a, s);
+}
+template <class T>
+void closure_features_w(bool odd, dim3 grid, const StepArgs *args, float *out, hipStream_t s) {
+  if (odd)
+    hipLaunchKernelGGL((k_closure_features<T, true>), grid, dim3(256), 0, s, args, out);
+  else
+    hipLaunchKernelGGL((k_closure_features<T, false>), grid, dim3(256), 0, s, args, out);
+}
+
+hipError_t launch_closure_features(int dtype, bool odd, const StepArgs *args, long long ncell, float *out,
+                                  hipStream_t s) {
+  const dim3 grid(static_cast<unsigned>((ncell + 255) / 256));
+  switch (dtype) {
+    case ASLHIP_F32: closure_features_w<float>(odd, grid, args, out, s); break;
+    case ASLHIP_BF16: closure_features_w<unsigned short>(odd, grid, args, out, s); break;
+    case ASLHIP_F16: closure_features_w<_Float16>(odd, grid, args, out, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 hipError_t launch_reduce(int q, int dtype, const ReduceArgs &a, double *out, hipStream_t s) {
   return dispatch<ReduceF>(q, dtype, a, out, s);

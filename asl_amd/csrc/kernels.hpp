@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "closure.hpp"
 #include "lattice_consts.hpp"
 
 namespace aslhip {
@@ -16,7 +17,8 @@ enum : unsigned {
   STEP_F_RAW = 4u,           // read ghost links raw (no BC resolution)
   STEP_F_COMPRESSIBLE = 8u,  // compressible equilibrium (ASL default)
   STEP_F_OMEGA = 16u,        // rotating frame: momentum += (2/w) omega x momentum
-  STEP_F_COUNT = 32u,        // number of flag combinations
+  STEP_F_CLOSURE = 32u,      // per-cell relaxation time from the closure network
+  STEP_F_COUNT = 64u,        // number of flag combinations
 };
 
 // One fused collide+stream pass (AA pattern) over the ghost-inclusive block.
@@ -46,6 +48,8 @@ struct StepArgs {
   double fval64[6][27];       // the same for FP64 lattices
   float om32[3];              // angular velocity of the frame (STEP_F_OMEGA)
   double om64[3];
+  const ClosureParams *cl;    // closure network (STEP_F_CLOSURE)
+  float cl_tau0;              // 3 nu + 1/2 in float
   const unsigned char *code;  // packed cell types
   const unsigned short *pat;  // link pattern of CODE_FLUID_BC / CODE_GHOST cells
   const LinkRec *rec;         // link patterns
@@ -121,6 +125,7 @@ struct ReduceArgs {
 struct StepLaunch {
   const StepArgs *args;
   int nblocks;
+  bool closure;  // the closure variant of the kernel (D3Q19, FP32 collision)
 };
 
 // wide: the populations span more than 4 GiB (64-bit per-lane addresses).
@@ -128,6 +133,10 @@ hipError_t launch_step(int q, int dtype, bool odd, bool periodic, bool wide, con
 hipError_t launch_step_batch(int q, int dtype, bool odd, bool periodic, bool wide, const BatchLaunch &b,
                              hipStream_t s);
 hipError_t launch_vlinks(int q, int dtype, const VLinkArgs &a, hipStream_t s);
+// Closure features (7 floats per interior cell) of a periodic D3Q19 lattice
+// with FP32 collision; args: the lattice's StepArgs in device memory.
+hipError_t launch_closure_features(int dtype, bool odd, const StepArgs *args, long long ncell, float *out,
+                                  hipStream_t s);
 hipError_t launch_init(int q, int dtype, const InitArgs &a, hipStream_t s);
 hipError_t launch_unpermute(int q, int dtype, const UnpermuteArgs &a, hipStream_t s);
 // partial must hold 5 * reduce_blocks(len) doubles; out: 5 doubles (device).

@@ -26,14 +26,15 @@
 #include "../../include/aslhip.h"
 #include "kernels.hpp"
 #include "lattice_consts.hpp"
+#include "status.hpp"
 
 using namespace aslhip;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const char *fmt, ...) {
+int aslhip::fail(int code, const char *fmt, ...) {
   char buf[1024];
   va_list ap;
   va_start(ap, fmt);
@@ -43,11 +44,18 @@ int fail(int code, const char *fmt, ...) {
   return code;
 }
 
-#define HIPCHK(expr)                                                                      \
-  do {                                                                                    \
-    hipError_t e_ = (expr);                                                               \
-    if (e_ != hipSuccess) return fail(ASLHIP_EHIP, "ASL ERROR: %s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
+namespace {
+
+// Round to the nearest BF16 value (ties to even), kept as a float.
+float bf16_round(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return x;  // inf / nan unchanged
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
 
 struct Templ {
   int Q;
@@ -132,6 +140,10 @@ struct aslhip_lattice {
   bool compressible = true, comp_rho = true, comp_vel = true;
   bool has_omega = false;
   double omega[3] = {0., 0., 0.};
+  // learned closure (network parameters in device memory)
+  bool has_closure = false;
+  ClosureParams *d_cl = nullptr;
+  float *d_feat = nullptr;  // closure features scratch (7 per interior cell)
   double dx = 1., pos0[3] = {0., 0., 0.};
   // BCs and order
   std::vector<BCDesc> bcs;
@@ -176,7 +188,7 @@ void drop_graphs(aslhip_lattice *h) {
 void free_all(aslhip_lattice *h) {
   drop_graphs(h);
   void *ptrs[] = {h->f,  h->rho,  h->u,           h->code,         h->pat, h->rec, h->mode,
-                  h->val, h->vl, h->vlbc, h->red_partial, h->red_out, h->d_args};
+                  h->val, h->vl, h->vlbc, h->red_partial, h->red_out, h->d_args, h->d_cl, h->d_feat};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -186,7 +198,8 @@ void free_all(aslhip_lattice *h) {
 // (src/num/aslLBGK.cxx:203).
 // StepArgs::flags bits fixed by the lattice's parameters.
 unsigned base_flags(const aslhip_lattice *h) {
-  return (h->compressible ? STEP_F_COMPRESSIBLE : 0u) | (h->has_omega ? STEP_F_OMEGA : 0u);
+  return (h->compressible ? STEP_F_COMPRESSIBLE : 0u) | (h->has_omega ? STEP_F_OMEGA : 0u) |
+         (h->has_closure ? STEP_F_CLOSURE : 0u);
 }
 
 double relaxation(const aslhip_lattice *h) {
@@ -583,6 +596,8 @@ StepArgs step_args(const aslhip_lattice *h) {
     a.om32[d] = static_cast<float>(h->omega[d]);
     a.om64[d] = h->omega[d];
   }
+  a.cl = h->d_cl;
+  a.cl_tau0 = 3.f * static_cast<float>(h->nu) + .5f;
   a.flags = base_flags(h);
   return a;
 }
@@ -623,6 +638,7 @@ int launch_one(aslhip_lattice *h, unsigned fl, std::vector<hipEvent_t> *ev, int 
   StepLaunch a;
   a.nblocks = h->h_args[0].nblocks;
   a.args = h->d_args + fl;
+  a.closure = h->has_closure;
   if (ev) HIPCHK(hipEventRecord((*ev)[2 * s], h->stream));
   HIPCHK(launch_step(h->Q, h->dtype, h->next_odd, h->periodic, is_wide(h), a, h->stream));
   if (ev) HIPCHK(hipEventRecord((*ev)[2 * s + 1], h->stream));
@@ -839,6 +855,49 @@ int aslhip_set_omega(aslhip_lattice *h, const double omega[3]) {
   h->has_omega = omega != nullptr;
   for (int d = 0; d < 3; ++d) h->omega[d] = omega ? omega[d] : 0.;
   return 0;
+}
+
+int aslhip_closure_set(aslhip_lattice *h, const float *params, const float mu[7], const float sc[7], float out_scale) {
+  if (int r = check(h)) return r;
+  if (params && (h->Q != 19 || h->dtype == ASLHIP_F64))
+    return fail(ASLHIP_EINVAL, "ASL ERROR: the closure needs a D3Q19 lattice with FP32 collision");
+  if (params && (!mu || !sc || !std::isfinite(out_scale) || out_scale < 0.f))
+    return fail(ASLHIP_EINVAL, "ASL ERROR: closure_set: bad normalisation or output scale");
+  if (int r = flush_deferred(h)) return r;  // a pending step runs with the old relaxation
+  if (!params) {
+    h->has_closure = false;
+    return 0;
+  }
+  ClosureParams c;
+  for (int i = 0; i < CL_NPARAM; ++i) {
+    if (!std::isfinite(params[i])) return fail(ASLHIP_EINVAL, "ASL ERROR: closure_set: non-finite parameter %d", i);
+    c.p[i] = bf16_round(params[i]);
+  }
+  for (int j = 0; j < CL_NIN; ++j) {
+    c.mu[j] = mu[j];
+    c.sc[j] = sc[j];
+  }
+  c.out = out_scale;
+  if (!h->d_cl) HIPCHK(hipMalloc(&h->d_cl, sizeof(ClosureParams)));
+  HIPCHK(hipStreamSynchronize(h->stream));  // steps queued earlier read the old parameters
+  HIPCHK(hipMemcpy(h->d_cl, &c, sizeof c, hipMemcpyHostToDevice));
+  h->has_closure = true;
+  return 0;
+}
+
+int aslhip_closure_features(aslhip_lattice *h, float *out) {
+  if (int r = check(h)) return r;
+  if (!out) return fail(ASLHIP_EINVAL, "ASL ERROR: closure_features needs an output buffer");
+  if (h->Q != 19 || h->dtype == ASLHIP_F64 || !h->periodic)
+    return fail(ASLHIP_EINVAL, "ASL ERROR: closure features need a periodic D3Q19 lattice with FP32 collision");
+  if (!h->initialized) return fail(ASLHIP_ESTATE, "ASL ERROR: closure_features before initF");
+  if (int r = flush_deferred(h)) return r;
+  if (int r = prepare(h, false)) return r;
+  if (int r = upload_args(h)) return r;
+  const long long ncell = (long long)h->n[0] * h->n[1] * h->n[2];
+  if (!h->d_feat) HIPCHK(hipMalloc(&h->d_feat, (size_t)ncell * CL_NIN * sizeof(float)));
+  HIPCHK(launch_closure_features(h->dtype, h->next_odd, h->d_args, ncell, h->d_feat, h->stream));
+  return copy_out(h, h->d_feat, out, (size_t)ncell * CL_NIN * sizeof(float));
 }
 
 int aslhip_set_compute_macro(aslhip_lattice *h, int comp_rho, int comp_vel) {
@@ -1185,6 +1244,7 @@ int batch_step_impl(aslhip_batch *b, int nsteps, unsigned flags) {
     aslhip_lattice *h = b->lat[l];
     if (int r = prepare(h, false)) return r;
     if (h->nvl) return fail(ASLHIP_EINVAL, "ASL ERROR: velocity map BCs are not supported in batches");
+    if (h->has_closure) return fail(ASLHIP_EINVAL, "ASL ERROR: the closure is not supported in batches");
     if (h->next_odd != odd) return fail(ASLHIP_ESTATE, "ASL ERROR: batch lattices are at different step parities");
     desc[l] = step_args(h);
     desc[n + l] = desc[l];

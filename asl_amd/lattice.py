@@ -67,6 +67,25 @@ class Lattice:
     def set_compressible(self, flag=True):
         call("aslhip_set_compressible", self.h, 1 if flag else 0)
 
+    def set_closure(self, params, mu=None, sc=None, out_scale=1.0):
+        """Learned closure: network parameters (1345 floats, BF16-rounded inside),
+        input shift mu / scale sc (7 each) and output scale; None: plain LBGK."""
+        if params is None:
+            call("aslhip_closure_set", self.h, None, None, None, 0.0)
+            return
+        fp = ctypes.POINTER(ctypes.c_float)
+        arrs = [np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1))
+                for a in (params, np.zeros(7) if mu is None else mu, np.ones(7) if sc is None else sc)]
+        if arrs[0].size != _lib.CLOSURE_NPARAM or arrs[1].size != 7 or arrs[2].size != 7:
+            raise ValueError("closure: 1345 parameters and 7-vectors mu, sc expected")
+        call("aslhip_closure_set", self.h, *[a.ctypes.data_as(fp) for a in arrs], float(out_scale))
+
+    def closure_features(self):
+        """Network inputs (float32, shape size + (7,)) of every interior cell for the next step."""
+        out = np.empty(tuple(self.size) + (7,), dtype=np.float32)
+        call("aslhip_closure_features", self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return out
+
     def set_omega(self, omega):
         """LBGK::setOmega: angular velocity (3 components) of the rotating frame; None turns it off."""
         if omega is None:
@@ -164,6 +183,52 @@ class Lattice:
         out = (ctypes.c_int64 * 4)()
         call("aslhip_stats", self.h, out)
         return {"counted": out[0], "computed": out[1], "boundary": out[2], "patterns": out[3]}
+
+
+class Mlp:
+    """Trainer of the closure network on the GPU (hand-written HIP kernels)."""
+
+    def __init__(self, params):
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float32).reshape(-1))
+        if p.size != _lib.CLOSURE_NPARAM:
+            raise ValueError("1345 parameters expected")
+        h = ctypes.c_void_p()
+        call("aslhip_mlp_create", ctypes.byref(h), p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        self.h = h
+        self.n = 0
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._LIB is not None:
+            _lib._LIB.aslhip_mlp_destroy(self.h)
+            self.h = None
+
+    def set_data(self, x, t):
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(-1, 7))
+        t = np.ascontiguousarray(np.asarray(t, dtype=np.float32).reshape(-1))
+        if x.shape[0] != t.size:
+            raise ValueError("one target per sample expected")
+        fp = ctypes.POINTER(ctypes.c_float)
+        call("aslhip_mlp_set_data", self.h, x.ctypes.data_as(fp), t.ctypes.data_as(fp), int(t.size))
+        self.n = int(t.size)
+
+    def gradient(self, first=0, count=None):
+        count = self.n - first if count is None else count
+        g = np.empty(_lib.CLOSURE_NPARAM, dtype=np.float32)
+        loss = ctypes.c_double(0)
+        call("aslhip_mlp_gradient", self.h, int(first), int(count), g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+             ctypes.byref(loss))
+        return loss.value, g
+
+    def train(self, steps, batch, lr, seed=1):
+        losses = np.empty(max(int(steps), 1))
+        call("aslhip_mlp_train", self.h, int(steps), int(batch), float(lr), int(seed),
+             losses.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return losses[:steps]
+
+    def params(self):
+        p = np.empty(_lib.CLOSURE_NPARAM, dtype=np.float32)
+        call("aslhip_mlp_get_params", self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return p
 
 
 class Batch:

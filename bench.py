@@ -9,17 +9,25 @@ no-slip side walls; initial u = (0.04, 0, 0) + 1e-3 * U[-1, 1] per cell and
 component from a seeded generator, u = 0 inside the sphere.  A step is one
 fused collide+stream pass over the whole lattice with every BC applied.
 
-The timed region holds exactly --steps steps with the populations already in
-HBM; it is bracketed by stream synchronisations.  ``value`` is MLUPS over
-computation nodes (solid and ghost cells are not counted).  The roofline
+The timed region holds exactly --steps steps (hipGraph replays of 100-step
+chunks) with the populations already in HBM; it is bracketed by stream
+synchronisations.  ``value`` is MLUPS over computation nodes (solid and
+ghost cells are not counted) from the wall clock around it.  The roofline
 entry is the step kernel's algorithmic HBM bytes per launch over its average
-launch duration, measured with HIP events recorded on the lattice's stream.
+launch duration, measured afterwards with HIP events recorded around each of
+up to 1000 eager launches on the lattice's stream.
 ``cpu_baseline`` times the CPU oracle (the C++ restatement of ASL's LBGK) on
 a bounded sample of the same lattice with the host's cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --copy-calibration BYTES   (one plain device copy, for
        scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters)
+       python bench.py --precision-study N [--snapshots 500,2000,5000]
+           (D3Q27 periodic Taylor-Green vortex on N^3, tau = 0.5005: FP32, FP16
+           and BF16 storage against FP64; one JSON line per storage type)
+       python bench.py --batch-study [--steps K]
+           (64 lid-driven cavities of log-uniform sizes 32..160 and tau in
+           [0.51, 1]: one batched launch per step vs one launch per lattice)
 """
 import argparse
 import json
@@ -76,6 +84,24 @@ def setup_fields():
     return mapv, u
 
 
+def tgv_fields(n, u0=0.04):
+    """3D Taylor-Green vortex on a periodic n^3 lattice (ASL layout, float64).
+
+    u = u0 sin kx cos ky cos kz, v = -u0 cos kx sin ky cos kz, w = 0 with
+    k = 2 pi / n at x = index - 1, and the matching pressure
+    p = u0^2 / 16 (cos 2kx + cos 2ky)(cos 2kz + 2), rho = 1 + 3 p.
+    """
+    k = 2.0 * np.pi / n
+    x = (np.arange(n + 2, dtype=np.float64) - 1.0) * k
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    u = np.empty((3,) + X.shape)
+    u[0] = u0 * np.sin(X) * np.cos(Y) * np.cos(Z)
+    u[1] = -u0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+    u[2] = 0.0
+    rho = 1.0 + 3.0 * (u0 * u0 / 16.0) * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0)
+    return rho, u
+
+
 def build_gpu(mapv, u):
     import asl_amd
     from asl_amd import _lib
@@ -128,6 +154,101 @@ def copy_calibration(nbytes):
     print("copied %d bytes device to device" % (a.numel() * 4), flush=True)
 
 
+def precision_study(n, snapshots, u0=0.04, tau=0.5005):
+    """Storage precision vs FP64 on the D3Q27 Taylor-Green vortex (one GPU).
+
+    The FP64 run goes first and keeps u at every snapshot on the host; each
+    other storage type is then run and compared: max|u - u64| / u0, the
+    relative KE drift against FP64, and KE(t) / KE(0).
+    """
+    import asl_amd
+    nu = (tau - 0.5) / 3.0
+    rho, u = tgv_fields(n, u0)
+    inner = (slice(None),) + (slice(1, -1),) * 3
+    ref = {}
+    for dt in ("f64", "f32", "f16", "bf16"):
+        L = asl_amd.Lattice((n, n, n), 27, dtype=dt, periodic=True, nu=nu)
+        L.init_equilibrium(rho, u)
+        ke0 = L.conservation()[4]
+        done, t0, rows = 0, time.perf_counter(), []
+        for snap in snapshots:
+            L.step(snap - done)
+            done = snap
+            ke = L.conservation()[4]
+            v = L.velocity()[inner].astype(np.float64)
+            if dt == "f64":
+                ref[snap] = (v, ke / ke0)
+                rows.append({"step": snap, "ke_ratio": ke / ke0})
+            else:
+                rows.append({"step": snap, "ke_ratio": ke / ke0,
+                             "max_du_over_u0": float(np.abs(v - ref[snap][0]).max() / u0),
+                             "rms_du_over_u0": float(np.sqrt(np.mean((v - ref[snap][0]) ** 2)) / u0),
+                             "ke_ratio_minus_fp64": ke / ke0 - ref[snap][1]})
+            del v
+            print("# %s step %d done (%.1f s)" % (dt, snap, time.perf_counter() - t0), flush=True)
+        del L
+        print(json.dumps({"study": "storage precision, D3Q27 periodic Taylor-Green vortex", "n": n, "tau": tau,
+                          "u0": u0, "storage": dt, "snapshots": rows}), flush=True)
+
+
+def batch_specs(n=64, seed=SEED):
+    """(L, tau) of the batched-cavities workload: L = round(exp(U[ln 32, ln 160])), tau ~ U[0.51, 1]."""
+    rng = np.random.default_rng(seed)
+    L = np.rint(np.exp(rng.uniform(np.log(32.0), np.log(160.0), n))).astype(int)
+    tau = rng.uniform(0.51, 1.0, n)
+    return [(int(a), float(b)) for a, b in zip(L, tau)]
+
+
+def batch_cavity(n, tau, lid=0.05):
+    """Lid-driven cavity n^3 (lid u = (lid, 0, 0) on ZE executed first, no-slip elsewhere)."""
+    import asl_amd
+    from asl_amd import _lib
+    L = asl_amd.Lattice((n, n, n), 19, dtype="f32", nu=(tau - 0.5) / 3.0)
+    L.init_equilibrium(1.0, (0.0, 0.0, 0.0))
+    bcs = [L.add_slice_bc(_lib.BC_CONSTANT_VELOCITY, _lib.ZE, u=(lid, 0.0, 0.0)),
+           L.add_slice_bc(_lib.BC_NOSLIP, _lib.X0 | _lib.XE | _lib.Y0 | _lib.YE | _lib.Z0)]
+    for b in bcs:
+        L.execute_bc(b)
+    return L
+
+
+def batch_study(steps, warmup):
+    """MLUPS of 64 uneven cavities: one batched launch per step (hipGraph chunks)
+    against the same lattices stepped one launch per lattice and step."""
+    import asl_amd
+    specs = batch_specs()
+    cells = sum(n ** 3 for n, _ in specs)
+    lat = [batch_cavity(n, t) for n, t in specs]
+    B = asl_amd.Batch(lat)
+    B.step(max(warmup, 1), write_macro=False)
+    ms_batch = B.time_steps(steps, graph=True)
+    solo = [batch_cavity(n, t) for n, t in specs]
+    for L in solo:
+        L.step(max(warmup, 1), write_macro=False)
+    for L in solo:
+        L.synchronize()
+    t0 = time.perf_counter()
+    for L in solo:
+        L.step(steps, write_macro=False)
+    for L in solo:
+        L.synchronize()
+    ms_solo = (time.perf_counter() - t0) * 1e3
+    # sequential single-lattice launches, one lattice after the other (ASL's pattern, without its waits)
+    t0 = time.perf_counter()
+    for _ in range(min(steps, 100)):
+        for L in solo:
+            L.step(1, write_macro=False)
+    for L in solo:
+        L.synchronize()
+    ms_interleaved = (time.perf_counter() - t0) * 1e3 * steps / min(steps, 100)
+    print(json.dumps({"study": "batched lattices", "lattices": len(specs), "cells": cells,
+                      "sizes": [n for n, _ in specs], "steps": steps,
+                      "batch_graph_mlups": round(cells * steps / ms_batch / 1e3, 1),
+                      "batch_gbps": round(152.0 * cells * steps / ms_batch / 1e6, 1),
+                      "per_lattice_streams_mlups": round(cells * steps / ms_solo / 1e3, 1),
+                      "per_lattice_interleaved_mlups": round(cells * steps / ms_interleaved / 1e3, 1)}), flush=True)
+
+
 def profile_traffic():
     """HBM bytes per launch of the step kernel from the committed rocprofv3 summary, or None."""
     path = os.path.join(ROOT, "profiles", "lbgk_step_d3q19_f32_sphere_256x128x128_traffic.md")
@@ -146,7 +267,18 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES bytes device to device once and exit (reference kernel of known "
                          "traffic for the HBM counters)")
+    ap.add_argument("--precision-study", type=int, default=0, metavar="N",
+                    help="only run the storage-precision study on an N^3 D3Q27 Taylor-Green vortex")
+    ap.add_argument("--snapshots", default="500,2000,5000", help="steps at which the precision study compares")
+    ap.add_argument("--batch-study", action="store_true",
+                    help="only run the batched-lattices study (64 cavities of uneven sizes)")
     args = ap.parse_args()
+    if args.batch_study:
+        batch_study(args.steps if args.steps != 5000 else 1000, args.warmup if args.warmup != 200 else 20)
+        return
+    if args.precision_study:
+        precision_study(args.precision_study, [int(x) for x in args.snapshots.split(",")])
+        return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -163,18 +295,22 @@ def main():
     counted = st["counted"]
     interior = int(np.prod(SIZE))
     if args.warmup:
-        L.step(args.warmup, write_macro=False)
+        L.step(args.warmup, write_macro=False, graph=True)
     L.synchronize()
+    # the timed steps: hipGraph-replayed chunks of 100 launches
     t0 = time.perf_counter()
-    ms_total, ms_kernel = L.time_steps(args.steps, write_macro=False)
+    ms_graph, _ = L.time_steps(args.steps, write_macro=False, graph=True)
     L.synchronize()
     wall = time.perf_counter() - t0
     mlups = counted * args.steps / wall / 1e6
+    # the step kernel's own duration: HIP events around each launch
+    nk = min(args.steps, 1000)
+    _, ms_kernel = L.time_steps(nk, write_macro=False)
 
     # algorithmic bytes per launch: 19 FP32 reads + 19 FP32 writes per
     # computation node, plus the one-byte cell type of every interior cell
     bytes_per_launch = 152.0 * counted + 1.0 * interior
-    kern_s = ms_kernel / args.steps / 1e3
+    kern_s = ms_kernel / nk / 1e3
     achieved = bytes_per_launch / kern_s / 1e9
     traffic = profile_traffic()
     line = {
@@ -196,7 +332,8 @@ def main():
                    "computation_nodes": counted, "storage": "FP32 SoA, in-place AA pattern"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel_ms_avg": round(kern_s * 1e3, 5), "bytes_per_launch": bytes_per_launch},
+                     "kernel_ms_avg": round(kern_s * 1e3, 5), "kernel_launches_timed": nk,
+                     "bytes_per_launch": bytes_per_launch, "graph_ms_per_step": round(ms_graph / args.steps, 5)},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -207,6 +207,40 @@ int aslhip_set_tuning(aslhip_lattice *h, int knob, int value);
 int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags,
                       double *ms_total, double *ms_kernel);
 
+/* ---- learned closure ------------------------------------------------
+ * Extension (no counterpart in ASL): a per-cell relaxation-time correction
+ * from a 7-32-32-1 network of the pre-collision non-equilibrium moments
+ *   x = (Pi_xx, Pi_yy, Pi_zz, Pi_xy, Pi_xz, Pi_yz, rho - 1),
+ *   Pi_ab = sum_i w_i c_ia c_ib f_i - rho/3 delta_ab - m_a m_b / rho,
+ *   xn = (x - mu) * sc, h1 = relu(W1 xn + b1), h2 = relu(W2 h1 + b2),
+ *   y = w3.h2 + b3, tau = 3 nu + 1/2 + max(y, 0) * out_scale.
+ * params (1345 floats): W1 (32x7, row per unit), b1, W2 (32x32), b2, w3, b3;
+ * rounded to BF16 and evaluated in FP32 inside the step kernel.  D3Q19
+ * lattices with FP32 collision (FP32, BF16 or FP16 storage); params NULL
+ * switches back to plain LBGK. */
+#define ASLHIP_CLOSURE_NPARAM 1345
+int aslhip_closure_set(aslhip_lattice *h, const float *params, const float mu[7], const float sc[7],
+                       float out_scale);
+/* The network inputs x of every interior cell for the next step (periodic
+ * D3Q19 lattices): out holds 7 floats per cell, cells in ASL order. */
+int aslhip_closure_features(aslhip_lattice *h, float *out);
+
+/* Trainer of the closure network on one GPU (FP32 master weights, Adam:
+ * Kingma & Ba 2015, beta1 0.9, beta2 0.999, eps 1e-8).  Loss: half the mean
+ * squared error of y against the targets. */
+typedef struct aslhip_mlp aslhip_mlp;
+int aslhip_mlp_create(aslhip_mlp **out, const float *params);
+int aslhip_mlp_destroy(aslhip_mlp *m);
+/* Training set: x (n x 7, already normalised: xn above) and targets t (n);
+ * copied to the device. */
+int aslhip_mlp_set_data(aslhip_mlp *m, const float *x, const float *t, long long n);
+/* Gradient (1345 floats) and loss over samples first .. first+count-1. */
+int aslhip_mlp_gradient(aslhip_mlp *m, long long first, int count, float *grad, double *loss);
+/* steps Adam steps on batches of `batch` samples drawn (with replacement) by
+ * a counter-based hash of seed; losses (may be NULL): loss of every batch. */
+int aslhip_mlp_train(aslhip_mlp *m, int steps, int batch, double lr, unsigned long long seed, double *losses);
+int aslhip_mlp_get_params(aslhip_mlp *m, float *out);
+
 /* ---- batches of independent lattices ---------------------------------
  * Extension (ASL advances each lattice with its own blocking launches):
  * many lattices of uneven sizes, each with its own viscosity and BCs, advance

@@ -34,6 +34,10 @@ def load_library():
     lib.orc_set_viscosity.argtypes = [ctypes.c_void_p, ctypes.c_double]
     lib.orc_set_compressible.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_set_omega.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.orc_set_closure.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]
+    lib.orc_set_closure.restype = ctypes.c_int
+    lib.orc_closure_features.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.orc_closure_features.restype = ctypes.c_int
     lib.orc_set_compute_macro.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     lib.orc_set_geometry.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
     lib.orc_init_equilibrium.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
@@ -111,6 +115,27 @@ class OracleLattice:
 
     def set_compressible(self, flag=True):
         self.lib.orc_set_compressible(self.h, 1 if flag else 0)
+
+    def set_closure(self, params, mu=None, sc=None, out_scale=1.0):
+        """Learned closure: 1345 network parameters (BF16-rounded inside), input
+        shift / scale (7 each) and output scale; params None turns it off."""
+        if params is None:
+            rc = self.lib.orc_set_closure(self.h, None, None, None, 0.0)
+        else:
+            self._cl = [np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1)) for a in
+                        (params, np.zeros(7) if mu is None else mu, np.ones(7) if sc is None else sc)]
+            assert self._cl[0].size == 1345 and self._cl[1].size == 7 and self._cl[2].size == 7
+            rc = self.lib.orc_set_closure(self.h, self._cl[0].ctypes.data, self._cl[1].ctypes.data,
+                                          self._cl[2].ctypes.data, float(out_scale))
+        if rc:
+            raise ValueError("the closure needs an FP32 D3Q19 lattice")
+
+    def closure_features(self):
+        """Features (7 per interior cell, float32, shape size + (7,)) of a periodic D3Q19 lattice."""
+        out = np.empty(tuple(self.size) + (7,), dtype=np.float32)
+        if self.lib.orc_closure_features(self.h, out.ctypes.data):
+            raise ValueError("closure features need a periodic D3Q19 lattice")
+        return out
 
     def set_omega(self, omega):
         """LBGK::setOmega: angular velocity (3 floats) of the rotating frame, None: off."""

@@ -30,6 +30,17 @@
 
 namespace {
 
+// Round to the nearest BF16 value (ties to even), kept as a float.
+float bf16_round(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return x;  // inf / nan unchanged
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+
 // ---------------------------------------------------------------------------
 // Lattice templates — src/math/aslTemplates.cxx.
 // ---------------------------------------------------------------------------
@@ -144,6 +155,12 @@ struct Lattice {
   // rotating frame, constants rounded to T like ASL's literals
   bool has_omega = false;
   T om[3] = {0, 0, 0};
+  // Learned closure (an extension; ASL has no counterpart): a 7-32-32-1
+  // network of the pre-collision moments gives a per-cell relaxation-time
+  // correction.  Parameters are BF16-rounded, evaluated in float.
+  bool has_closure = false;
+  std::vector<float> cl;          // CL_NPARAM values, layout of closure_eval
+  float cl_mu[7], cl_sc[7], cl_out = 0.f;
   double dx = 1., pos0[3] = {0., 0., 0.};
   std::vector<BC<T>> bcs;
   // per-direction constants as ASL prints them (double -> T)
@@ -208,6 +225,52 @@ struct Lattice {
   // Every x*y + z of ASL's expression tree is contracted to fma(x, y, z), as an
   // OpenCL compiler does under the default FP_CONTRACT ON; the GPU kernel
   // contracts the same sites, so both round identically.
+  // Features of one cell: the non-equilibrium second moments
+  // Pi_ab = sum_i w_i c_ia c_ib f_i - (rho/3 delta_ab + m_a m_b / rho) in the
+  // order xx, yy, zz, xy, xz, yz, then rho - 1.
+  inline void closure_features(const T *f, T r, const T *m, T *x) const {
+    static const int A[6] = {0, 1, 2, 0, 0, 1}, B[6] = {0, 1, 2, 1, 2, 2};
+    T mr[3];
+    for (int d = 0; d < 3; ++d) mr[d] = m[d] / r;
+    const T third = T(1.) / T(3.);
+    for (int p = 0; p < 6; ++p) {
+      bool first = true;
+      T acc = 0;
+      for (int j = 0; j < vt.Q; ++j) {
+        const int cc = vt.c[j][A[p]] * vt.c[j][B[p]];
+        if (cc == 0) continue;
+        const T wcc = T(vt.w[j] * cc);
+        acc = first ? f[j] * wcc : std::fma(f[j], wcc, acc);
+        first = false;
+      }
+      const T t = A[p] == B[p] ? std::fma(m[A[p]], mr[B[p]], r * third) : m[A[p]] * mr[B[p]];
+      x[p] = acc - t;
+    }
+    x[6] = r - T(1.);
+  }
+
+  // The network: h1 = relu(W1 xn + b1), h2 = relu(W2 h1 + b2), y = w3.h2 + b3
+  // with xn = (x - mu) * sc; the relaxation time becomes tau0 + max(y, 0) * out.
+  inline T closure_w(const T *x, T tau0) const {
+    const float *W1 = &cl[0], *b1 = &cl[224], *W2 = &cl[256], *b2 = &cl[1280], *w3 = &cl[1312];
+    float xn[7], h1[32], h2[32];
+    for (int j = 0; j < 7; ++j) xn[j] = (float(x[j]) - cl_mu[j]) * cl_sc[j];
+    for (int o = 0; o < 32; ++o) {
+      float a = b1[o];
+      for (int j = 0; j < 7; ++j) a = std::fma(W1[o * 7 + j], xn[j], a);
+      h1[o] = a > 0.f ? a : 0.f;
+    }
+    for (int o = 0; o < 32; ++o) {
+      float a = b2[o];
+      for (int j = 0; j < 32; ++j) a = std::fma(W2[o * 32 + j], h1[j], a);
+      h2[o] = a > 0.f ? a : 0.f;
+    }
+    float y = cl[1344];
+    for (int j = 0; j < 32; ++j) y = std::fma(w3[j], h2[j], y);
+    const float dtau = (y > 0.f ? y : 0.f) * cl_out;
+    return T(1.) / (tau0 + T(dtau));
+  }
+
   inline void collide(T *f, T w, T &rho_o, T *v_o) const {
     const int Q = vt.Q;
     // momentum = computeMomentum(fOld) (:51-63), pRho = computeRho(fOld) (:46-49)
@@ -224,6 +287,11 @@ struct Lattice {
     }
     T r = f[0] * cw[0];
     for (int j = 1; j < Q; ++j) r = std::fma(f[j], cw[j], r);
+    if (has_closure) {
+      T x[7];
+      closure_features(f, r, m, x);
+      w = closure_w(x, T(3.) * nu + T(.5));
+    }
     // momentum += (2./w) * crossProduct(omega, momentum) (:207-208).  The
     // three components are three statements of the generated kernel, in
     // order (operator+=, src/acl/aclMath/aclVectorOfElementsOperations.cxx:249-259;
@@ -585,6 +653,71 @@ void orc_set_omega(void *h, const double *om) {
     for (int d = 0; d < 3; ++d) L.om[d] = om ? T(om[d]) : T(0);
   });
 }
+int orc_set_closure(void *h, const float *params, const float *mu, const float *sc, float out) {
+  int rc = 0;
+  dispatch(h, [&](auto &L) {
+    if (sizeof(L.om[0]) != sizeof(float) || L.vt.Q != 19) {
+      rc = -1;
+      return;
+    }
+    L.has_closure = params != nullptr;
+    if (!params) return;
+    L.cl.resize(1345);
+    for (int i = 0; i < 1345; ++i) L.cl[i] = bf16_round(params[i]);
+    for (int j = 0; j < 7; ++j) {
+      L.cl_mu[j] = mu[j];
+      L.cl_sc[j] = sc[j];
+    }
+    L.cl_out = out;
+  });
+  return rc;
+}
+
+// Closure features of every interior cell (periodic lattices): the inputs the
+// next collision would see, 7 per cell, cells in ASL order.
+int orc_closure_features(void *h, float *out) {
+  int rc = 0;
+  dispatch(h, [&](auto &L) {
+    using T = std::remove_reference_t<decltype(L.om[0])>;
+    if (!L.periodic || L.vt.Q != 19) {
+      rc = -1;
+      return;
+    }
+    const int Q = L.vt.Q;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int x = 1; x <= L.N[0]; ++x)
+      for (int y = 1; y <= L.N[1]; ++y)
+        for (int z = 1; z <= L.N[2]; ++z) {
+          T f[27];
+          for (int i = 0; i < Q; ++i) {
+            int xs = x - L.vt.c[i][0], ys = y - L.vt.c[i][1], zs = z - L.vt.c[i][2];
+            xs = xs < 1 ? L.N[0] : (xs > L.N[0] ? 1 : xs);
+            ys = ys < 1 ? L.N[1] : (ys > L.N[1] ? 1 : ys);
+            zs = zs < 1 ? L.N[2] : (zs > L.N[2] ? 1 : zs);
+            f[i] = L.fper[(size_t)i * L.len + L.c2i(xs, ys, zs)];
+          }
+          T m[3];
+          for (int d = 0; d < 3; ++d) {
+            bool first = true;
+            T acc = 0;
+            for (int j = 0; j < Q; ++j) {
+              if (L.vt.c[j][d] == 0) continue;
+              acc = first ? f[j] * L.cwc[j][d] : std::fma(f[j], L.cwc[j][d], acc);
+              first = false;
+            }
+            m[d] = acc;
+          }
+          T r = f[0] * L.cw[0];
+          for (int j = 1; j < Q; ++j) r = std::fma(f[j], L.cw[j], r);
+          T xf[7];
+          L.closure_features(f, r, m, xf);
+          const size_t c = ((size_t)(x - 1) * L.N[1] + (y - 1)) * L.N[2] + (z - 1);
+          for (int p = 0; p < 7; ++p) out[7 * c + p] = float(xf[p]);
+        }
+  });
+  return rc;
+}
+
 void orc_set_compute_macro(void *h, int r, int v) {
   dispatch(h, [&](auto &L) {
     L.comp_rho = r != 0;

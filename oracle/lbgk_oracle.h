@@ -36,6 +36,12 @@ int orc_set_threads(int n);
 void orc_set_viscosity(void *h, double nu);
 void orc_set_compressible(void *h, int c);
 void orc_set_omega(void *h, const double *omega); /* NULL: off */
+/* Learned closure (FP32 D3Q19 lattices): 1345 parameters (BF16-rounded
+ * here), input shift/scale (7 each), output scale; params NULL: off.
+ * Returns -1 for other lattices. */
+int orc_set_closure(void *h, const float *params, const float *mu, const float *sc, float out);
+/* Closure features (7 per interior cell) of a periodic D3Q19 lattice. */
+int orc_closure_features(void *h, float *out);
 void orc_set_compute_macro(void *h, int comp_rho, int comp_vel);
 void orc_set_geometry(void *h, double dx, const double pos[3]);
 void orc_init_equilibrium(void *h, double rho, const double u[3]);
