@@ -20,6 +20,7 @@ STEP_GRAPH = 4
 GRAPH_CHUNK = 100
 CODE_FLUID, CODE_FLUID_BC, CODE_SKIP, CODE_GHOST = 0, 1, 2, 3
 TUNE_XCD_REMAP = 1
+TUNE_BLOCK = 2
 CLOSURE_NPARAM = 1345
 
 # Every entry point of include/aslhip.h with its ctypes signature.
@@ -52,6 +53,7 @@ SIGNATURES = {
     "aslhip_reduce_conservation": [_vp, _dp],
     "aslhip_stats": [_vp, ctypes.POINTER(ctypes.c_int64)],
     "aslhip_set_tuning": [_vp, ctypes.c_int, ctypes.c_int],
+    "aslhip_autotune": [_vp, ctypes.c_int, _ip, _ip],
     "aslhip_time_steps": [_vp, ctypes.c_int, ctypes.c_uint, _dp, _dp],
     "aslhip_batch_create": [ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int],
     "aslhip_batch_destroy": [_vp],

@@ -55,7 +55,7 @@ def turbulence_field(n, seed, u_rms=0.03, kpeak=4.0):
     uh[1] -= KY * div
     uh[2] -= KZ * div
     del div, KX, KY, KZ, k2, kk, amp
-    u = np.stack([np.fft.irfftn(c, s=(n, n, n)).astype(np.float32) for c in uh])
+    u = np.stack([np.fft.irfftn(c, s=(n, n, n), axes=(0, 1, 2)).astype(np.float32) for c in uh])
     del uh
     u -= u.mean(axis=(1, 2, 3), keepdims=True)
     u *= np.float32(u_rms / np.sqrt((u * u).sum(0).mean() / 3.0))
@@ -161,6 +161,17 @@ class Experiment:
         return np.array(out), time.perf_counter() - t0
 
 
+def init_params(seed=0):
+    """Random initial network parameters (1345 floats): uniform He-style
+    weights (bound sqrt(6 / fan_in), sqrt(3 / fan_in) for the output unit),
+    zero biases; layout W1 (32x7), b1, W2 (32x32), b2, w3, b3."""
+    rng = np.random.default_rng(seed)
+    W1 = rng.uniform(-1, 1, (32, 7)) * np.sqrt(6.0 / 7)
+    W2 = rng.uniform(-1, 1, (32, 32)) * np.sqrt(6.0 / 32)
+    w3 = rng.uniform(-1, 1, 32) * np.sqrt(3.0 / 32)
+    return np.concatenate([W1.ravel(), np.zeros(32), W2.ravel(), np.zeros(32), w3, [0.0]]).astype(np.float32)
+
+
 def normalisation(x, t):
     """Input shift / scale and output scale of a training set."""
     mu = x.mean(0).astype(np.float32)
@@ -180,5 +191,5 @@ def train(x, t, params0, steps=3000, batch=8192, lr=2e-3, seed=1):
     return m.params(), mu, sc, out, losses
 
 
-__all__ = ["Experiment", "turbulence_field", "box_filter", "eddy_viscosity", "strain_rate", "train",
+__all__ = ["Experiment", "turbulence_field", "box_filter", "eddy_viscosity", "strain_rate", "train", "init_params",
            "normalisation", "with_ghosts"]

@@ -222,7 +222,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L) {
   constexpr unsigned ES = sizeof(S);
   const int plane = L / a.bpp;
   const int chunk = L - plane * a.bpp;
-  const unsigned off = static_cast<unsigned>(chunk * STEP_BLOCK) + threadIdx.x;
+  const unsigned off = static_cast<unsigned>(chunk * a.block) + threadIdx.x;
   if (off >= static_cast<unsigned>(a.ny * a.sy)) return;
   // Row and column inside the plane (exact: off * sy < 2^40, see StepArgs::zmagic).
   const unsigned row = static_cast<unsigned>((static_cast<unsigned long long>(off) * a.zmagic) >> 40);
@@ -520,6 +520,16 @@ __global__ __launch_bounds__(BLOCK) void k_init(const InitArgs a) {
   for (int d = 0; d < 3; ++d) static_cast<M *>(a.u)[d * a.len + idx] = M(u[d]);
 }
 
+// Cell whose slot i holds population i of cell idx after an odd step: idx + c_i,
+// wrapped into the interior on periodic lattices.
+template <int Q>
+__device__ __forceinline__ int odd_slot(int idx, int i, int sx, int sy, int per, int nx, int ny, int nz) {
+  if (!per) return idx + offc<Q>(i, sx, sy);
+  const int x = idx / sx, rem = idx - x * sx, y = rem / sy, z = rem - y * sy;
+  if (x < 1 || x > nx || y < 1 || y > ny || z < 1 || z > nz) return idx + offc<Q>(i, sx, sy);
+  return wrap1(x + Lat<Q>::c[i][0], nx) * sx + wrap1(y + Lat<Q>::c[i][1], ny) * sy + wrap1(z + Lat<Q>::c[i][2], nz);
+}
+
 // The post-collision populations in ASL's view (f_i of cell x), from the AA
 // state.  Slots outside the array (outer ghost layer) read as NaN.
 template <int Q, class S>
@@ -533,7 +543,7 @@ __global__ __launch_bounds__(BLOCK) void k_unpermute(const UnpermuteArgs a) {
   for (int i = 0; i < Q; ++i) {
     M v;
     if (a.after_odd) {
-      const int src = idx + offc<Q>(i, a.sx, a.sy);
+      const int src = odd_slot<Q>(idx, i, a.sx, a.sy, a.per, a.nx, a.ny, a.nz);
       v = (src >= 0 && src < a.len)
               ? M(CD::ld(a.f + i * a.qs_bytes, static_cast<unsigned>(src) * sizeof(S)))
               : M(__builtin_nan(""));
@@ -560,7 +570,8 @@ __global__ __launch_bounds__(BLOCK) void k_reduce(const ReduceArgs a) {
     for (int i = 0; i < Q; ++i) {
       double fi;
       if (a.after_odd)
-        fi = double(CD::ld(a.f + i * a.qs_bytes, static_cast<unsigned>(idx + offc<Q>(i, a.sx, a.sy)) * sizeof(S)));
+        fi = double(CD::ld(a.f + i * a.qs_bytes,
+                           static_cast<unsigned>(odd_slot<Q>(idx, i, a.sx, a.sy, a.per, a.nx, a.ny, a.nz)) * sizeof(S)));
       else
         fi = double(CD::ld(a.f + inv_dir<Q>(i) * a.qs_bytes, static_cast<unsigned>(idx) * sizeof(S)));
       fi *= Lat<Q>::w(i);
@@ -607,7 +618,7 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_final(const double *partial, i
 
 template <int Q, class S, bool ODD, bool PER>
 hipError_t step_w(bool wide, const StepLaunch &a, hipStream_t s) {
-  const dim3 grid(a.nblocks), block(STEP_BLOCK);
+  const dim3 grid(a.nblocks), block(a.block);
   if (a.closure) {
     // the closure variants exist for D3Q19 with FP32 collision only
     if constexpr (Q == 19 && sizeof(typename Codec<S>::C) == 4) {

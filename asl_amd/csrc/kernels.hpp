@@ -39,7 +39,8 @@ struct StepArgs {
   int sx, sy;                 // strides of the layout: Sy*Sz and Sz
   int nx, ny, nz;             // interior size
   int len;                    // ghost-inclusive cell count
-  int bpp;                    // STEP_BLOCK-cell chunks per plane
+  int block;                  // workgroup size (a multiple of 64, at most STEP_BLOCK)
+  int bpp;                    // block-cell chunks per plane
   int nblocks;                // bpp * nx
   unsigned long long zmagic;  // ceil(2^40 / sy): row = (off * zmagic) >> 40
   int remap;                  // 1: XCD-contiguous chunk order (see xcd_remap)
@@ -104,10 +105,13 @@ struct InitArgs {
   void *rho, *u;
 };
 
+// per: periodic lattice of interior size nx x ny x nz (after an odd step the
+// populations of a face cell sit in the wrapped neighbour's slots).
 struct UnpermuteArgs {
   const char *f;
   long long qs_bytes;
   int sx, sy, len;
+  int per, nx, ny, nz;
   int after_odd;
   void *out;  // Q * len macroscopic type
 };
@@ -116,6 +120,7 @@ struct ReduceArgs {
   const char *f;
   long long qs_bytes;
   int sx, sy, len;
+  int per, nx, ny, nz;
   int after_odd;
   const unsigned char *code;
   double *partial;  // 5 per block
@@ -125,6 +130,7 @@ struct ReduceArgs {
 struct StepLaunch {
   const StepArgs *args;
   int nblocks;
+  int block;
   bool closure;  // the closure variant of the kernel (D3Q19, FP32 collision)
 };
 

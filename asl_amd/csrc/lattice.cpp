@@ -158,7 +158,8 @@ struct aslhip_lattice {
   bool deferred_raw = false;
   bool deferred_first = false;
   // launch tuning
-  int tune_remap = 1;
+  int tune_block = STEP_BLOCK;
+  int tune_remap = 0;  // XCD-contiguous chunk order off: measured slower (fewer open DRAM pages shared)
   // face rule: per face slot * 4 + mode of the winning slice BC, and values
   int fface[6] = {0, 0, 0, 0, 0, 0};
   float fval32[6][27] = {};
@@ -562,7 +563,7 @@ int run_vlinks(aslhip_lattice *h, bool after_odd) {
 bool is_wide(const aslhip_lattice *h) { return h->qs_bytes * h->Q > 0xffffffffLL; }
 
 // Kernel arguments of one step of h (flags filled in by the caller).
-StepArgs step_args(const aslhip_lattice *h) {
+StepArgs step_args(const aslhip_lattice *h, int block = 0) {
   StepArgs a;
   std::memset(&a, 0, sizeof a);  // compared bytewise by upload_args
   for (int k = 0; k < 27; ++k) a.fd[k] = k < h->Q ? h->f + k * h->qs_bytes : nullptr;
@@ -580,7 +581,8 @@ StepArgs step_args(const aslhip_lattice *h) {
   a.ny = h->n[1];
   a.nz = h->n[2];
   a.len = h->len;
-  a.bpp = (h->n[1] * h->sy + STEP_BLOCK - 1) / STEP_BLOCK;
+  a.block = block > 0 ? block : h->tune_block;
+  a.bpp = (h->n[1] * h->sy + a.block - 1) / a.block;
   a.nblocks = a.bpp * h->n[0];
   a.zmagic = ((1ULL << 40) + (unsigned long long)h->sy - 1) / (unsigned long long)h->sy;
   a.remap = h->tune_remap;
@@ -637,6 +639,7 @@ int upload_args(aslhip_lattice *h) {
 int launch_one(aslhip_lattice *h, unsigned fl, std::vector<hipEvent_t> *ev, int s) {
   StepLaunch a;
   a.nblocks = h->h_args[0].nblocks;
+  a.block = h->h_args[0].block;
   a.args = h->d_args + fl;
   a.closure = h->has_closure;
   if (ev) HIPCHK(hipEventRecord((*ev)[2 * s], h->stream));
@@ -1061,6 +1064,10 @@ int aslhip_read_f(aslhip_lattice *h, void *out) {
   a.sx = h->sx;
   a.sy = h->sy;
   a.len = h->len;
+  a.per = h->periodic ? 1 : 0;
+  a.nx = h->n[0];
+  a.ny = h->n[1];
+  a.nz = h->n[2];
   a.after_odd = h->next_odd ? 0 : 1;
   a.out = d;
   HIPCHK(launch_unpermute(h->Q, h->dtype, a, h->stream));
@@ -1091,6 +1098,10 @@ int aslhip_reduce_conservation(aslhip_lattice *h, double out[5]) {
   a.sx = h->sx;
   a.sy = h->sy;
   a.len = h->len;
+  a.per = h->periodic ? 1 : 0;
+  a.nx = h->n[0];
+  a.ny = h->n[1];
+  a.nz = h->n[2];
   a.after_odd = h->next_odd ? 0 : 1;
   a.code = h->code;
   a.partial = h->red_partial;
@@ -1117,7 +1128,41 @@ int aslhip_set_tuning(aslhip_lattice *h, int knob, int value) {
     h->tune_remap = value ? 1 : 0;
     return 0;
   }
+  if (knob == ASLHIP_TUNE_BLOCK) {
+    if (value < 64 || value > STEP_BLOCK || value % 64)
+      return fail(ASLHIP_EINVAL, "ASL ERROR: workgroup size %d is not a multiple of 64 in 64..%d", value, STEP_BLOCK);
+    if (int r = flush_deferred(h)) return r;
+    if (value != h->tune_block) drop_graphs(h);  // captured launches carry the old grid
+    h->tune_block = value;
+    return 0;
+  }
   return fail(ASLHIP_EINVAL, "ASL ERROR: unknown tuning knob %d", knob);
+}
+
+int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_remap) {
+  if (int r = check(h)) return r;
+  if (nsteps < 1) return fail(ASLHIP_EINVAL, "ASL ERROR: autotune needs at least one step per candidate");
+  nsteps += nsteps & 1;
+  const int blocks[3] = {256, 512, 1024};
+  double best = 1e300;
+  int bb = h->tune_block, br = h->tune_remap;
+  for (int remap = 0; remap < 2; ++remap)
+    for (int blk : blocks) {
+      if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_XCD_REMAP, remap)) return r;
+      if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_BLOCK, blk)) return r;
+      double tot = 0., ker = 0.;
+      if (int r = aslhip_time_steps(h, nsteps, 0u, &tot, &ker)) return r;
+      if (ker < best) {
+        best = ker;
+        bb = blk;
+        br = remap;
+      }
+    }
+  if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_XCD_REMAP, br)) return r;
+  if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_BLOCK, bb)) return r;
+  if (best_block) *best_block = bb;
+  if (best_remap) *best_remap = br;
+  return 0;
 }
 
 int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags, double *ms_total, double *ms_kernel) {
@@ -1246,7 +1291,7 @@ int batch_step_impl(aslhip_batch *b, int nsteps, unsigned flags) {
     if (h->nvl) return fail(ASLHIP_EINVAL, "ASL ERROR: velocity map BCs are not supported in batches");
     if (h->has_closure) return fail(ASLHIP_EINVAL, "ASL ERROR: the closure is not supported in batches");
     if (h->next_odd != odd) return fail(ASLHIP_ESTATE, "ASL ERROR: batch lattices are at different step parities");
-    desc[l] = step_args(h);
+    desc[l] = step_args(h, STEP_BLOCK);  // batch launches use the full workgroup
     desc[n + l] = desc[l];
     if (h->comp_rho) desc[n + l].flags |= STEP_F_RHO;
     if (h->comp_vel) desc[n + l].flags |= STEP_F_VEL;
@@ -1296,7 +1341,7 @@ int aslhip_batch_create(aslhip_batch **out, aslhip_lattice *const *lattices, int
       delete b;
       return fail(ASLHIP_EINVAL, "ASL ERROR: batch lattices must share template, population type and periodicity");
     }
-    const StepArgs a = step_args(h);
+    const StepArgs a = step_args(h, STEP_BLOCK);
     first[l] = static_cast<int>(blat.size());
     blat.insert(blat.end(), a.nblocks, static_cast<unsigned short>(l));
   }

@@ -145,6 +145,12 @@ class Lattice:
     def set_tuning(self, knob, value):
         call("aslhip_set_tuning", self.h, int(knob), int(value))
 
+    def autotune(self, nsteps=20):
+        """Time every launch setting over nsteps real steps and keep the fastest: (block, remap)."""
+        b, r = ctypes.c_int(0), ctypes.c_int(0)
+        call("aslhip_autotune", self.h, int(nsteps), ctypes.byref(b), ctypes.byref(r))
+        return b.value, r.value
+
     def time_steps(self, nsteps, write_macro=False, graph=False):
         """(ms of the whole call, summed ms of the step kernels; 0 with graph=True)."""
         tot = ctypes.c_double(0)

@@ -25,6 +25,10 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --precision-study N [--snapshots 500,2000,5000]
            (D3Q27 periodic Taylor-Green vortex on N^3, tau = 0.5005: FP32, FP16
            and BF16 storage against FP64; one JSON line per storage type)
+       python bench.py --closure-study N [--tau-fine T] [--chunk C] [--train-snapshots K]
+           [--eval-chunks E]  (learned closure: N^3 coarse vs (4N)^3 fine decaying
+           turbulence; KE(t) of fine, coarse and coarse+closure, wall times)
+       python bench.py --cavity-study   (the 64^3 lid-driven cavity: GPU and CPU-oracle MLUPS)
        python bench.py --batch-study [--steps K]
            (64 lid-driven cavities of log-uniform sizes 32..160 and tau in
            [0.51, 1]: one batched launch per step vs one launch per lattice)
@@ -191,6 +195,45 @@ def precision_study(n, snapshots, u0=0.04, tau=0.5005):
                           "u0": u0, "storage": dt, "snapshots": rows}), flush=True)
 
 
+def cavity_study(no_cpu=False, steps=1000):
+    """64^3 lid-driven cavity (tau 0.6, lid 0.05 on ZE executed first, no-slip
+    elsewhere), FP32: GPU MLUPS and step-kernel bandwidth, CPU oracle MLUPS."""
+    n = 64
+    L = batch_cavity(n, 0.6)
+    L.step(100, write_macro=False, graph=True)
+    L.synchronize()
+    t0 = time.perf_counter()
+    L.time_steps(steps, write_macro=False, graph=True)
+    L.synchronize()
+    wall = time.perf_counter() - t0
+    _, ms_kernel = L.time_steps(steps, write_macro=False)
+    cells = n ** 3
+    kern_s = ms_kernel / steps / 1e3
+    line = {"study": "64^3 lid-driven cavity, D3Q19 FP32", "steps": steps,
+            "gpu_mlups": round(cells * steps / wall / 1e6, 1), "ms_per_step": round(wall * 1e3 / steps, 5),
+            "kernel_ms_avg": round(kern_s * 1e3, 5),
+            "kernel_gbps": round((152.0 * cells + cells) / kern_s / 1e9, 1)}
+    if not no_cpu:
+        import oracle
+        from oracle.lbgk import set_threads
+        threads = set_threads(min(16, os.cpu_count() or 1))
+        O = oracle.OracleLattice((n, n, n), 19, fp64=False, nu=(0.6 - 0.5) / 3.0)
+        O.init_f(1.0, (0.0, 0.0, 0.0))
+        bcs = [O.add_slice_bc(O.CONSTANT_VELOCITY, 32, u=(0.05, 0.0, 0.0)), O.add_slice_bc(O.NOSLIP, 31)]
+        for b in bcs:
+            O.execute_bc(b)
+        O.run(5, bcs)
+        k, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and k < steps:
+            O.run(10, bcs)
+            k += 10
+        el = time.perf_counter() - t0
+        line["cpu_oracle_mlups"] = round(cells * k / el / 1e6, 2)
+        line["cpu_cores"] = threads
+        line["cpu_sample_steps"] = k
+    print(json.dumps(line), flush=True)
+
+
 def batch_specs(n=64, seed=SEED):
     """(L, tau) of the batched-cavities workload: L = round(exp(U[ln 32, ln 160])), tau ~ U[0.51, 1]."""
     rng = np.random.default_rng(seed)
@@ -249,6 +292,44 @@ def batch_study(steps, warmup):
                       "per_lattice_interleaved_mlups": round(cells * steps / ms_interleaved / 1e3, 1)}), flush=True)
 
 
+def closure_study(nc, tau_fine, chunk, snaps, eval_chunks):
+    """Coarse N^3 with and without the learned closure against the 4x finer run."""
+    from asl_amd import closure
+    t0 = time.perf_counter()
+    ex = closure.Experiment(n_coarse=nc, tau_fine=tau_fine, seed=SEED)
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    log("# closure study: coarse %d^3, fine %d^3, tau_fine %.4f; field ready (%.1f s)"
+        % (nc, ex.nf, tau_fine, time.perf_counter() - t0))
+    x, t = ex.training_data(chunk, snaps, log=log)
+    tt = time.perf_counter()
+    params, mu, sc, out, losses = closure.train(x, t, closure.init_params(1), steps=4000, batch=16384, lr=2e-3)
+    train_s = time.perf_counter() - tt
+    log("# trained on %d samples in %.1f s: loss %.4f -> %.4f" % (t.size, train_s, losses[:20].mean(),
+                                                                 losses[-20:].mean()))
+    del x, t
+    F = ex.fine()
+    ke_f, wall_f = ex.ke_history(F, chunk, eval_chunks, fine=True)
+    del F
+    log("# fine run done (%.1f s)" % wall_f)
+    P = ex.coarse()
+    ke_p, wall_p = ex.ke_history(P, chunk, eval_chunks)
+    C = ex.coarse()
+    C.set_closure(params, mu, sc, out)
+    ke_c, wall_c = ex.ke_history(C, chunk, eval_chunks)
+
+    def err(ke):
+        return float(np.abs((ke / ke[0]) / (ke_f / ke_f[0]) - 1.0).max())
+    print(json.dumps({"study": "learned closure, D3Q19 decaying turbulence", "coarse": nc, "fine": ex.nf,
+                      "tau_fine": tau_fine, "tau_coarse": 0.5 + 3 * ex.nu_c, "coarse_steps_per_chunk": chunk,
+                      "train_snapshots": snaps, "eval_chunks": eval_chunks,
+                      "ke_fine": ke_f.tolist(), "ke_coarse_plain": ke_p.tolist(), "ke_coarse_closure": ke_c.tolist(),
+                      "max_rel_ke_decay_error_plain": err(ke_p), "max_rel_ke_decay_error_closure": err(ke_c),
+                      "wall_s_fine": wall_f, "wall_s_coarse_plain": wall_p, "wall_s_coarse_closure": wall_c,
+                      "speedup_closure_vs_fine": wall_f / wall_c, "train_s": train_s,
+                      "train_loss_first": float(losses[:20].mean()), "train_loss_last": float(losses[-20:].mean())}),
+          flush=True)
+
+
 def profile_traffic():
     """HBM bytes per launch of the step kernel from the committed rocprofv3 summary, or None."""
     path = os.path.join(ROOT, "profiles", "lbgk_step_d3q19_f32_sphere_256x128x128_traffic.md")
@@ -270,9 +351,23 @@ def main():
     ap.add_argument("--precision-study", type=int, default=0, metavar="N",
                     help="only run the storage-precision study on an N^3 D3Q27 Taylor-Green vortex")
     ap.add_argument("--snapshots", default="500,2000,5000", help="steps at which the precision study compares")
+    ap.add_argument("--closure-study", type=int, default=0, metavar="N",
+                    help="only run the learned-closure study with an N^3 coarse lattice")
+    ap.add_argument("--tau-fine", type=float, default=0.52)
+    ap.add_argument("--chunk", type=int, default=50, help="coarse steps between snapshots")
+    ap.add_argument("--train-snapshots", type=int, default=8)
+    ap.add_argument("--eval-chunks", type=int, default=16)
+    ap.add_argument("--cavity-study", action="store_true",
+                    help="only measure the 64^3 lid-driven cavity (GPU and CPU oracle)")
     ap.add_argument("--batch-study", action="store_true",
                     help="only run the batched-lattices study (64 cavities of uneven sizes)")
     args = ap.parse_args()
+    if args.cavity_study:
+        cavity_study(args.no_cpu)
+        return
+    if args.closure_study:
+        closure_study(args.closure_study, args.tau_fine, args.chunk, args.train_snapshots, args.eval_chunks)
+        return
     if args.batch_study:
         batch_study(args.steps if args.steps != 5000 else 1000, args.warmup if args.warmup != 200 else 20)
         return

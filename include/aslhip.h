@@ -193,12 +193,20 @@ int aslhip_reduce_conservation(aslhip_lattice *h, double out[5]);
  * cells, distinct link patterns. */
 int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
 
-/* Launch tuning of the step kernel (no effect on results).  The autotuner
- * in asl_amd/autotune.py measures the knobs and keeps the fastest setting.
+/* Launch tuning of the step kernel (no effect on results).
  *   ASLHIP_TUNE_XCD_REMAP: 1 = each XCD takes one contiguous range of the
- *   lattice (default), 0 = hardware round-robin order. */
+ *   lattice, 0 = hardware round-robin order (default).
+ *   ASLHIP_TUNE_BLOCK: workgroup size, a multiple of 64 up to 1024 (default
+ *   1024); a workgroup takes that many consecutive cells of a plane. */
 #define ASLHIP_TUNE_XCD_REMAP 1
+#define ASLHIP_TUNE_BLOCK 2
 int aslhip_set_tuning(aslhip_lattice *h, int knob, int value);
+
+/* The autotuner: times nsteps (rounded up to even) real steps of the lattice
+ * for every candidate setting (XCD remap 0/1 x workgroup 256/512/1024) with
+ * HIP events, keeps the fastest and returns it (either pointer may be NULL).
+ * The trial steps advance the simulation like any other steps. */
+int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_remap);
 
 /* Times nsteps steps with HIP events recorded on the lattice's stream.
  * ms_total: wall time of the whole call between the events; ms_kernel:

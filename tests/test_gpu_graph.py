@@ -27,13 +27,13 @@ def cavity(n=20):
     return L, bcs
 
 
-def rotor(n=24):
-    size = (n, n, 8)
-    mapv = -geometry.normalize(geometry.cylinder_inf(size, 5.0, (0, 0, 1), (n / 2, n / 2, 4)))
+def rotor():
+    size = (20, 16, 14)
+    body = geometry.normalize(geometry.sphere(size, 3.5, (9.0, 7.5, 6.5)))
     L = asl_amd.Lattice(size, 19, nu=0.05)
     L.init_equilibrium(1.0, (0, 0, 0))
-    bcs = [L.add_map_bc(_lib.BC_VELOCITY_MAP, mapv, u=(0, 0, 0), rot=(0, 0, 0.01, n / 2, n / 2, 4)),
-           L.add_slice_bc(_lib.BC_NOSLIP, WALLS | _lib.ZE)]
+    bcs = [L.add_slice_bc(_lib.BC_NOSLIP, 63),
+           L.add_map_bc(_lib.BC_VELOCITY_MAP, body, u=(0.02, 0, 0.01), rot=(0, 0, 0.01, 9.0, 7.5, 6.5))]
     for b in bcs:
         L.execute_bc(b)
     return L, bcs
@@ -81,3 +81,23 @@ def test_timed_graph_steps():
     G.step(1)
     E.step(1)
     same(G, E)
+
+
+def test_launch_tuning_and_autotune_keep_results():
+    """Workgroup size and chunk order change the launch only, never the result."""
+    ref, _ = cavity()
+    ref.step(120)
+    for block, remap in ((256, 0), (512, 1), (64, 0)):
+        L, _ = cavity()
+        L.set_tuning(_lib.TUNE_BLOCK, block)
+        L.set_tuning(_lib.TUNE_XCD_REMAP, remap)
+        L.step(70, graph=True)
+        L.step(50)
+        same(L, ref)
+    L, _ = cavity()
+    best = L.autotune(10)  # 6 candidates x 10 real steps
+    assert best[0] in (256, 512, 1024) and best[1] in (0, 1)
+    L.step(120 - 60)
+    same(L, ref)
+    with pytest.raises(asl_amd.ASLError):
+        L.set_tuning(_lib.TUNE_BLOCK, 100)
