@@ -11,9 +11,10 @@ lattice produced by this same library:
 * at each snapshot the fine velocity is box-filtered by 4; the subgrid
   stress tau_ij = <u_i u_j> - <u_i><u_j> is projected on the filtered strain
   rate S_ij (central differences) to an eddy viscosity
-  nu_t = max(0, -<tau^d : S>_3 / (2 <S : S>_3)) with <.>_3 a 3^3 box average
-  (the least-squares estimate of the dynamic Smagorinsky procedure), and the
-  target relaxation-time correction is 3 nu_t;
+  nu_t = -<tau^d : S>_3 / (2 <S : S>_3) with <.>_3 a 3^3 box average (the
+  least-squares estimate of the dynamic Smagorinsky procedure; negative
+  values, backscatter, are kept down to -nu_coarse / 2), and the target
+  relaxation-time correction is 3 nu_t;
 * the coarse lattice, started from the filtered fields and run two steps,
   gives the network inputs of every cell.
 
@@ -62,6 +63,25 @@ def turbulence_field(n, seed, u_rms=0.03, kpeak=4.0):
     return u
 
 
+def pressure_field(u):
+    """Pressure of an incompressible periodic field: lap p = -d_i d_j (u_i u_j), spectrally
+    (float32, mean zero); the lattice density is then rho = 1 + 3 p."""
+    n = u.shape[-1]
+    k1 = np.fft.fftfreq(n, d=1.0 / n).astype(np.float32) * np.float32(2 * np.pi / n)
+    kz = np.fft.rfftfreq(n, d=1.0 / n).astype(np.float32) * np.float32(2 * np.pi / n)
+    K = np.meshgrid(k1, k1, kz, indexing="ij")
+    k2 = K[0] * K[0] + K[1] * K[1] + K[2] * K[2]
+    k2[0, 0, 0] = 1.0
+    ph = np.zeros(K[0].shape, dtype=np.complex64)
+    for i in range(3):
+        for j in range(i, 3):
+            w = 1.0 if i == j else 2.0
+            ph += np.float32(w) * K[i] * K[j] * np.fft.rfftn(u[i] * u[j], axes=(0, 1, 2))
+    ph = -ph / k2
+    ph[0, 0, 0] = 0.0
+    return np.fft.irfftn(ph, s=(n, n, n), axes=(0, 1, 2)).astype(np.float32)
+
+
 def with_ghosts(a):
     """Periodic ghost layer around the last three axes (ASL's ghost-inclusive layout)."""
     return np.pad(a, [(0, 0)] * (a.ndim - 3) + [(1, 1)] * 3, mode="wrap")
@@ -92,8 +112,8 @@ def strain_rate(u):
                      0.5 * (d[1][2] + d[2][1])])
 
 
-def eddy_viscosity(u_fine, r=4):
-    """Least-squares eddy viscosity of the box-filtered field (coarse lattice units)."""
+def eddy_viscosity(u_fine, r=4, floor=-np.inf):
+    """Least-squares eddy viscosity of the box-filtered field (coarse lattice units), at least floor."""
     ub = box_filter(u_fine, r)
     pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
     tau = np.stack([box_filter(u_fine[i] * u_fine[j], r) - ub[i] * ub[j] for i, j in pairs])
@@ -103,7 +123,7 @@ def eddy_viscosity(u_fine, r=4):
     wts = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0]).reshape(6, 1, 1, 1)  # off-diagonal pairs count twice
     num = box3((wts * tau * S).sum(0))
     den = box3((wts * S * S).sum(0))
-    return np.maximum(0.0, -num / (2.0 * np.maximum(den, 1e-30))), ub
+    return np.maximum(floor, -num / (2.0 * np.maximum(den, 1e-30))), ub
 
 
 class Experiment:
@@ -116,16 +136,19 @@ class Experiment:
         self.nu_c = self.nu_f / r
         self.seed, self.u_rms = seed, u_rms
         self.u0 = turbulence_field(self.nf, seed, u_rms)
+        # the pressure that goes with the field, so neither lattice starts with an acoustic transient
+        self.rho0 = np.float32(1.0) + np.float32(3.0) * pressure_field(self.u0)
 
     def coarse(self, u=None, rho=None):
         L = Lattice((self.nc,) * 3, 19, dtype="f32", periodic=True, nu=self.nu_c)
         ub = box_filter(self.u0) if u is None else u
-        L.init_equilibrium(1.0 if rho is None else with_ghosts(rho), with_ghosts(ub))
+        rb = box_filter(self.rho0) if rho is None else rho
+        L.init_equilibrium(with_ghosts(rb.astype(np.float32)), with_ghosts(ub.astype(np.float32)))
         return L
 
     def fine(self):
         L = Lattice((self.nf,) * 3, 19, dtype="f32", periodic=True, nu=self.nu_f)
-        L.init_equilibrium(1.0, with_ghosts(self.u0))
+        L.init_equilibrium(with_ghosts(self.rho0), with_ghosts(self.u0))
         return L
 
     def training_data(self, coarse_steps_between, snapshots, log=print):
@@ -138,7 +161,7 @@ class Experiment:
             done = snap * coarse_steps_between
             u = F.velocity()[inner]
             rho = box_filter(F.rho()[inner[1:]])
-            nut, ub = eddy_viscosity(u, self.r)
+            nut, ub = eddy_viscosity(u, self.r, floor=-0.5 * self.nu_c)
             del u
             C = self.coarse(ub.astype(np.float32), rho.astype(np.float32))
             C.step(2)

@@ -6,7 +6,8 @@
 // for ab = xx, yy, zz, xy, xz, yz, and rho - 1.  Network 7-32-32-1, ReLU
 // after both hidden layers, BF16-rounded parameters evaluated in FP32:
 //   xn = (x - mu) * sc,  h1 = relu(W1 xn + b1),  h2 = relu(W2 h1 + b2),
-//   y = w3 . h2 + b3,    tau = tau0 + max(y, 0) * out,  w = 1 / tau.
+//   y = w3 . h2 + b3,    tau = max(tau0 + y * out, 1/2 + (tau0 - 1/2) / 2),  w = 1 / tau
+// (the correction may lower the viscosity by at most half).
 // Every sum is an fma chain in index order (the CPU oracle evaluates the same
 // chain, so results agree bit for bit).
 #pragma once
@@ -71,8 +72,9 @@ __device__ __forceinline__ float closure_w(const float (&x)[7], const ClosurePar
     const float h2 = a > 0.f ? a : 0.f;
     y = __builtin_fmaf(c.p[CL_W3 + o], h2, y);
   }
-  const float dtau = (y > 0.f ? y : 0.f) * c.out;
-  return 1.f / (tau0 + dtau);
+  const float tau = tau0 + y * c.out;
+  const float lo = 0.5f + 0.5f * (tau0 - 0.5f);
+  return 1.f / (tau < lo ? lo : tau);
 }
 
 }  // namespace aslhip

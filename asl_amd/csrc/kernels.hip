@@ -192,12 +192,16 @@ __device__ __forceinline__ void collide(C (&v)[Q], C w, bool compressible, bool 
   for (int d = 0; d < 3; ++d) m_o[d] = m[d];
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (id % 8).  Give each
-// XCD one contiguous range of chunks so that neighbouring chunks, which share
-// cache lines at their edges, meet in the same L2.
-__device__ __forceinline__ int xcd_remap(int L, int n) {
-  const int q = n >> 3, r = n & 7, j = L & 7, t = L >> 3;
-  return (j < r) ? j * (q + 1) + t : r * (q + 1) + (j - r) * q + t;
+// Workgroups are dispatched round-robin over the 8 XCDs (id % 8).  With a
+// group size g > 1, each XCD takes runs of g consecutive chunks (workgroup b
+// of a group of 8 g works on chunk  group * 8g + (b % 8) * g + (b % 8g) / 8),
+// so that neighbouring chunks, which share cache lines at their edges, meet
+// in the same L2; the last, partial group keeps the hardware order.
+__device__ __forceinline__ int xcd_remap(int L, int n, int g) {
+  const int span = 8 * g, grp = L / span;
+  if ((grp + 1) * span > n) return L;
+  const int r = L - grp * span;
+  return grp * span + (r & 7) * g + (r >> 3);
 }
 
 __device__ __forceinline__ int wrap1(int v, int n) { return v < 1 ? n : (v > n ? 1 : v); }
@@ -387,7 +391,8 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L) {
 template <int Q, class S, bool ODD, bool PER, bool WIDE, bool CL>
 __global__ __launch_bounds__(STEP_BLOCK) void k_step(const StepArgs *args) {
   const StepArgs &a = *args;
-  step_body<Q, S, ODD, PER, WIDE, CL>(a, a.remap ? xcd_remap(blockIdx.x, a.nblocks) : static_cast<int>(blockIdx.x));
+  const int b = static_cast<int>(blockIdx.x);
+  step_body<Q, S, ODD, PER, WIDE, CL>(a, a.remap > 1 ? xcd_remap(b, a.nblocks, a.remap) : b);
 }
 
 // Many lattices in one launch: block b works on lattice lat[b], chunk

@@ -43,7 +43,7 @@ struct StepArgs {
   int bpp;                    // block-cell chunks per plane
   int nblocks;                // bpp * nx
   unsigned long long zmagic;  // ceil(2^40 / sy): row = (off * zmagic) >> 40
-  int remap;                  // 1: XCD-contiguous chunk order (see xcd_remap)
+  int remap;                  // > 1: runs of that many chunks per XCD (see xcd_remap)
   int fface[6];               // per face: slot * 4 + mode of its winning slice BC
   float fval32[6][27];        // per face and direction: addend / constant (FP32 lattices)
   double fval64[6][27];       // the same for FP64 lattices

@@ -159,7 +159,7 @@ struct aslhip_lattice {
   bool deferred_first = false;
   // launch tuning
   int tune_block = STEP_BLOCK;
-  int tune_remap = 0;  // XCD-contiguous chunk order off: measured slower (fewer open DRAM pages shared)
+  int tune_remap = 0;  // chunk runs per XCD (0: hardware order; the autotuner may choose others)
   // face rule: per face slot * 4 + mode of the winning slice BC, and values
   int fface[6] = {0, 0, 0, 0, 0, 0};
   float fval32[6][27] = {};
@@ -1125,7 +1125,8 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]) {
 int aslhip_set_tuning(aslhip_lattice *h, int knob, int value) {
   if (int r = check(h)) return r;
   if (knob == ASLHIP_TUNE_XCD_REMAP) {
-    h->tune_remap = value ? 1 : 0;
+    if (value < 0 || value > 4096) return fail(ASLHIP_EINVAL, "ASL ERROR: XCD run length %d outside 0..4096", value);
+    h->tune_remap = value;
     return 0;
   }
   if (knob == ASLHIP_TUNE_BLOCK) {
@@ -1146,7 +1147,7 @@ int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_re
   const int blocks[3] = {256, 512, 1024};
   double best = 1e300;
   int bb = h->tune_block, br = h->tune_remap;
-  for (int remap = 0; remap < 2; ++remap)
+  for (int remap : {0, 4, 16, 64})
     for (int blk : blocks) {
       if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_XCD_REMAP, remap)) return r;
       if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_BLOCK, blk)) return r;

@@ -194,8 +194,9 @@ int aslhip_reduce_conservation(aslhip_lattice *h, double out[5]);
 int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
 
 /* Launch tuning of the step kernel (no effect on results).
- *   ASLHIP_TUNE_XCD_REMAP: 1 = each XCD takes one contiguous range of the
- *   lattice, 0 = hardware round-robin order (default).
+ *   ASLHIP_TUNE_XCD_REMAP: g > 1 = each of the 8 XCDs takes runs of g
+ *   consecutive chunks (neighbouring chunks share one L2); 0 or 1 = the
+ *   hardware round-robin order (default).
  *   ASLHIP_TUNE_BLOCK: workgroup size, a multiple of 64 up to 1024 (default
  *   1024); a workgroup takes that many consecutive cells of a plane. */
 #define ASLHIP_TUNE_XCD_REMAP 1
@@ -203,7 +204,7 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
 int aslhip_set_tuning(aslhip_lattice *h, int knob, int value);
 
 /* The autotuner: times nsteps (rounded up to even) real steps of the lattice
- * for every candidate setting (XCD remap 0/1 x workgroup 256/512/1024) with
+ * for every candidate setting (XCD runs 0/4/16/64 x workgroup 256/512/1024) with
  * HIP events, keeps the fastest and returns it (either pointer may be NULL).
  * The trial steps advance the simulation like any other steps. */
 int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_remap);
@@ -221,7 +222,8 @@ int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags,
  *   x = (Pi_xx, Pi_yy, Pi_zz, Pi_xy, Pi_xz, Pi_yz, rho - 1),
  *   Pi_ab = sum_i w_i c_ia c_ib f_i - rho/3 delta_ab - m_a m_b / rho,
  *   xn = (x - mu) * sc, h1 = relu(W1 xn + b1), h2 = relu(W2 h1 + b2),
- *   y = w3.h2 + b3, tau = 3 nu + 1/2 + max(y, 0) * out_scale.
+ *   y = w3.h2 + b3, tau = max(tau0 + y * out_scale, 1/2 + (tau0 - 1/2) / 2)
+ *   with tau0 = 3 nu + 1/2 (the correction lowers the viscosity by at most half).
  * params (1345 floats): W1 (32x7, row per unit), b1, W2 (32x32), b2, w3, b3;
  * rounded to BF16 and evaluated in FP32 inside the step kernel.  D3Q19
  * lattices with FP32 collision (FP32, BF16 or FP16 storage); params NULL

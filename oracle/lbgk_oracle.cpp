@@ -250,7 +250,8 @@ struct Lattice {
   }
 
   // The network: h1 = relu(W1 xn + b1), h2 = relu(W2 h1 + b2), y = w3.h2 + b3
-  // with xn = (x - mu) * sc; the relaxation time becomes tau0 + max(y, 0) * out.
+  // with xn = (x - mu) * sc; the relaxation time becomes tau0 + y * out, at
+  // least 1/2 + (tau0 - 1/2) / 2.
   inline T closure_w(const T *x, T tau0) const {
     const float *W1 = &cl[0], *b1 = &cl[224], *W2 = &cl[256], *b2 = &cl[1280], *w3 = &cl[1312];
     float xn[7], h1[32], h2[32];
@@ -267,8 +268,9 @@ struct Lattice {
     }
     float y = cl[1344];
     for (int j = 0; j < 32; ++j) y = std::fma(w3[j], h2[j], y);
-    const float dtau = (y > 0.f ? y : 0.f) * cl_out;
-    return T(1.) / (tau0 + T(dtau));
+    const float tau = float(tau0) + y * cl_out;
+    const float lo = 0.5f + 0.5f * (float(tau0) - 0.5f);
+    return T(1.) / T(tau < lo ? lo : tau);
   }
 
   inline void collide(T *f, T w, T &rho_o, T *v_o) const {

@@ -87,17 +87,18 @@ def test_launch_tuning_and_autotune_keep_results():
     """Workgroup size and chunk order change the launch only, never the result."""
     ref, _ = cavity()
     ref.step(120)
-    for block, remap in ((256, 0), (512, 1), (64, 0)):
+    for block, remap in ((256, 0), (512, 16), (64, 3), (1024, 4096)):
         L, _ = cavity()
         L.set_tuning(_lib.TUNE_BLOCK, block)
         L.set_tuning(_lib.TUNE_XCD_REMAP, remap)
         L.step(70, graph=True)
         L.step(50)
         same(L, ref)
+    ref.step(120)
     L, _ = cavity()
-    best = L.autotune(10)  # 6 candidates x 10 real steps
-    assert best[0] in (256, 512, 1024) and best[1] in (0, 1)
-    L.step(120 - 60)
+    best = L.autotune(10)  # 12 candidates x 10 real steps
+    assert best[0] in (256, 512, 1024) and best[1] in (0, 4, 16, 64)
+    L.step(240 - 120)
     same(L, ref)
     with pytest.raises(asl_amd.ASLError):
         L.set_tuning(_lib.TUNE_BLOCK, 100)

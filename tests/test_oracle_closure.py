@@ -82,7 +82,8 @@ def closure_step_numpy(f, c, w, nu, params, mu, sc, out):
     x, rho, m = numpy_features(pulled, c, w)
     xn = (x - mu) * sc
     y, _ = mlp.forward(mlp.bf16_round(params).astype(np.float64), xn.reshape(-1, 7))
-    tau = 3.0 * nu + 0.5 + np.maximum(y, 0.0).reshape(rho.shape) * out
+    tau0 = 3.0 * nu + 0.5
+    tau = np.maximum(tau0 + y.reshape(rho.shape) * out, 0.5 + 0.5 * (tau0 - 0.5))
     wr = 1.0 / tau
     outf = np.empty_like(pulled)
     for i in range(19):
@@ -127,11 +128,19 @@ def test_closure_step_matches_numpy(random_state):
 
 def test_zero_correction_is_plain_bgk(random_state):
     size, f0 = random_state
-    p = np.zeros(mlp.NPARAM, np.float32)
-    p[mlp.NPARAM - 1] = -1.0  # y < 0 everywhere: no correction
+    p = np.zeros(mlp.NPARAM, np.float32)  # y = 0 everywhere: no correction
     plain = run_oracle(size, f0, 0.02)
     cl = run_oracle(size, f0, 0.02, (p, np.zeros(7), np.ones(7), 0.1))
     assert np.array_equal(plain, cl)
+
+
+def test_negative_correction_is_floored_at_half_the_viscosity(random_state):
+    size, f0 = random_state
+    p = np.zeros(mlp.NPARAM, np.float32)
+    p[mlp.NPARAM - 1] = -1.0  # tau0 - 1 is below the floor 1/2 + (tau0 - 1/2) / 2
+    cl = run_oracle(size, f0, 0.02, (p, np.zeros(7), np.ones(7), 1.0))
+    plain = run_oracle(size, f0, 0.01)
+    assert np.abs(cl - plain).max() < 1e-6
 
 
 def test_constant_correction_is_bgk_with_larger_tau(random_state):
