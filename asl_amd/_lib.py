@@ -49,6 +49,8 @@ SIGNATURES = {
     "aslhip_read_rho": [_vp, _vp],
     "aslhip_read_velocity": [_vp, _vp],
     "aslhip_read_f": [_vp, _vp],
+    "aslhip_write_f": [_vp, _vp],
+    "aslhip_template": [ctypes.c_int, _ip, _dp, _ip],
     "aslhip_read_cell_types": [_vp, _vp],
     "aslhip_reduce_conservation": [_vp, _dp],
     "aslhip_stats": [_vp, ctypes.POINTER(ctypes.c_int64)],
@@ -59,6 +61,8 @@ SIGNATURES = {
     "aslhip_batch_destroy": [_vp],
     "aslhip_batch_step": [_vp, ctypes.c_int, ctypes.c_uint],
     "aslhip_batch_time_steps": [_vp, ctypes.c_int, ctypes.c_uint, _dp],
+    "aslhip_batch_set_tuning": [_vp, ctypes.c_int, ctypes.c_int],
+    "aslhip_batch_autotune": [_vp, ctypes.c_int, _ip],
     "aslhip_closure_set": [_vp, _fp, _fp, _fp, ctypes.c_float],
     "aslhip_closure_features": [_vp, _fp],
     "aslhip_mlp_create": [ctypes.POINTER(_vp), _fp],

@@ -82,6 +82,29 @@ def pressure_field(u):
     return np.fft.irfftn(ph, s=(n, n, n), axes=(0, 1, 2)).astype(np.float32)
 
 
+def populations(rho, u, tau, q=19):
+    """ASL-normalised populations f_i / w_i of a periodic field: the equilibrium
+    rho + 3 c.m + 4.5 (c.m)^2 / rho - 1.5 m.m / rho (m = rho u) plus the first
+    Chapman-Enskog term -3 tau rho (c_a c_b - delta_ab / 3) d_a u_b (central
+    differences), so a run starts with the stress of its velocity field."""
+    from .lattice import VectorTemplate
+    t = VectorTemplate(q)
+    rho = np.asarray(rho, dtype=np.float64)
+    u = np.asarray(u, dtype=np.float64)
+    m = rho * u
+    mm = (m * m).sum(0) / rho
+    grad = [[(np.roll(u[b], -1, axis=a) - np.roll(u[b], 1, axis=a)) * 0.5 for b in range(3)] for a in range(3)]
+    div = grad[0][0] + grad[1][1] + grad[2][2]
+    f = np.empty((q,) + rho.shape, dtype=np.float32)
+    for i in range(q):
+        c = t.c[i]
+        cm = c[0] * m[0] + c[1] * m[1] + c[2] * m[2]
+        cgc = sum(c[a] * c[b] * grad[a][b] for a in range(3) for b in range(3) if c[a] * c[b])
+        neq = -3.0 * tau * rho * (cgc - div / 3.0)
+        f[i] = rho + 3.0 * cm + 4.5 * cm * cm / rho - 1.5 * mm + neq
+    return f
+
+
 def with_ghosts(a):
     """Periodic ghost layer around the last three axes (ASL's ghost-inclusive layout)."""
     return np.pad(a, [(0, 0)] * (a.ndim - 3) + [(1, 1)] * 3, mode="wrap")
@@ -143,12 +166,12 @@ class Experiment:
         L = Lattice((self.nc,) * 3, 19, dtype="f32", periodic=True, nu=self.nu_c)
         ub = box_filter(self.u0) if u is None else u
         rb = box_filter(self.rho0) if rho is None else rho
-        L.init_equilibrium(with_ghosts(rb.astype(np.float32)), with_ghosts(ub.astype(np.float32)))
+        L.write_f(with_ghosts(populations(rb, ub, 3.0 * self.nu_c + 0.5)))
         return L
 
     def fine(self):
         L = Lattice((self.nf,) * 3, 19, dtype="f32", periodic=True, nu=self.nu_f)
-        L.init_equilibrium(with_ghosts(self.rho0), with_ghosts(self.u0))
+        L.write_f(with_ghosts(populations(self.rho0, self.u0, 3.0 * self.nu_f + 0.5)))
         return L
 
     def training_data(self, coarse_steps_between, snapshots, log=print):
@@ -172,16 +195,34 @@ class Experiment:
         return np.concatenate(xs), np.concatenate(ts)
 
     def ke_history(self, lattice, steps_between, count, fine=False):
-        """Mean kinetic energy per cell after each of `count` chunks (coarse-step units)."""
+        """Mean kinetic energy per cell after each of `count` chunks (coarse-step units)
+        and the wall time of the stepping alone."""
         mult = self.r if fine else 1
         ncell = float((self.nf if fine else self.nc) ** 3)
         out = [lattice.conservation()[4] / ncell]
-        t0 = time.perf_counter()
+        wall = 0.0
         for _ in range(count):
+            lattice.synchronize()
+            t0 = time.perf_counter()
             lattice.step(steps_between * mult, write_macro=False, graph=True)
+            lattice.synchronize()
+            wall += time.perf_counter() - t0
             out.append(lattice.conservation()[4] / ncell)
-        lattice.synchronize()
-        return np.array(out), time.perf_counter() - t0
+        return np.array(out), wall
+
+    def filtered_ke_history(self, steps_between, count):
+        """Kinetic energy of the box-filtered fine run, per coarse cell: mean |<rho u>|^2 / (2 <rho>)."""
+        F = self.fine()
+        inner = (slice(None),) + (slice(1, -1),) * 3
+        out = []
+        for c in range(count + 1):
+            if c:
+                F.step(steps_between * self.r, write_macro=True, graph=True)
+            rho = F.rho()[inner[1:]].astype(np.float64)
+            m = box_filter(F.velocity()[inner].astype(np.float64) * rho, self.r)
+            rb = box_filter(rho, self.r)
+            out.append(float(((m * m).sum(0) / (2.0 * rb)).mean()))
+        return np.array(out)
 
 
 def init_params(seed=0):

@@ -398,11 +398,9 @@ __global__ __launch_bounds__(STEP_BLOCK) void k_step(const StepArgs *args) {
 // Many lattices in one launch: block b works on lattice lat[b], chunk
 // b - first[lat[b]] of it; descriptors live in device memory.
 template <int Q, class S, bool ODD, bool PER, bool WIDE>
-__global__ __launch_bounds__(STEP_BLOCK) void k_step_batch(const StepArgs *desc, const unsigned short *lat,
-                                                           const int *first) {
-  const int l = lat[blockIdx.x];
-  const StepArgs &a = desc[l];
-  step_body<Q, S, ODD, PER, WIDE>(a, static_cast<int>(blockIdx.x) - first[l]);
+__global__ __launch_bounds__(STEP_BLOCK) void k_step_batch(const StepArgs *desc, const uint2 *ent) {
+  const uint2 e = ent[blockIdx.x];
+  step_body<Q, S, ODD, PER, WIDE>(desc[e.x], static_cast<int>(e.y));
 }
 
 // Closure features of every interior cell of a periodic D3Q19 lattice: the
@@ -525,6 +523,27 @@ __global__ __launch_bounds__(BLOCK) void k_init(const InitArgs a) {
   for (int d = 0; d < 3; ++d) static_cast<M *>(a.u)[d * a.len + idx] = M(u[d]);
 }
 
+template <int Q, class S>
+__global__ __launch_bounds__(BLOCK) void k_write_f(const WriteFArgs a) {
+  using CD = Codec<S>;
+  using C = typename CD::C;
+  using M = typename CD::M;
+  const int idx = blockIdx.x * BLOCK + threadIdx.x;
+  if (idx >= a.len) return;
+  const M *in = static_cast<const M *>(a.in);
+  C v[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    v[i] = C(in[static_cast<long long>(i) * a.len + idx]);
+    CD::st(a.f + inv_dir<Q>(i) * a.qs_bytes, static_cast<unsigned>(idx) * sizeof(S), v[i]);
+  }
+  C r, m[3];
+  moments<Q, C>(v, r, m);
+  static_cast<M *>(a.rho)[idx] = M(r);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) static_cast<M *>(a.u)[d * a.len + idx] = M(a.compressible ? m[d] / r : m[d]);
+}
+
 // Cell whose slot i holds population i of cell idx after an odd step: idx + c_i,
 // wrapped into the interior on periodic lattices.
 template <int Q>
@@ -645,11 +664,11 @@ hipError_t step_w(bool wide, const StepLaunch &a, hipStream_t s) {
 
 template <int Q, class S, bool ODD, bool PER>
 hipError_t batch_w(bool wide, const BatchLaunch &b, hipStream_t s) {
-  const dim3 grid(b.nblocks), block(STEP_BLOCK);
+  const dim3 grid(b.nblocks), block(b.block);
   if (wide)
-    hipLaunchKernelGGL((k_step_batch<Q, S, ODD, PER, true>), grid, block, 0, s, b.desc, b.lat, b.first);
+    hipLaunchKernelGGL((k_step_batch<Q, S, ODD, PER, true>), grid, block, 0, s, b.desc, b.ent);
   else
-    hipLaunchKernelGGL((k_step_batch<Q, S, ODD, PER, false>), grid, block, 0, s, b.desc, b.lat, b.first);
+    hipLaunchKernelGGL((k_step_batch<Q, S, ODD, PER, false>), grid, block, 0, s, b.desc, b.ent);
   return hipGetLastError();
 }
 
@@ -707,6 +726,13 @@ struct InitF {
   }
 };
 template <int Q, class S>
+struct WriteFF {
+  static hipError_t run(const WriteFArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL((k_write_f<Q, S>), dim3((a.len + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+  }
+};
+template <int Q, class S>
 struct UnpermuteF {
   static hipError_t run(const UnpermuteArgs &a, hipStream_t s) {
     hipLaunchKernelGGL((k_unpermute<Q, S>), dim3((a.len + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, a);
@@ -741,6 +767,9 @@ hipError_t launch_vlinks(int q, int dtype, const VLinkArgs &a, hipStream_t s) {
   return dispatch<VLinkF>(q, dtype, a, s);
 }
 hipError_t launch_init(int q, int dtype, const InitArgs &a, hipStream_t s) { return dispatch<InitF>(q, dtype, a, s); }
+hipError_t launch_write_f(int q, int dtype, const WriteFArgs &a, hipStream_t s) {
+  return dispatch<WriteFF>(q, dtype, a, s);
+}
 hipError_t launch_unpermute(int q, int dtype, const UnpermuteArgs &a, hipStream_t s) {
   return dispatch<UnpermuteF>(q, dtype, a, s);
 }

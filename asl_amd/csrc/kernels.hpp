@@ -66,9 +66,9 @@ struct StepArgs {
 // chunk b - first[lat[b]].
 struct BatchLaunch {
   const StepArgs *desc;
-  const unsigned short *lat;
-  const int *first;
+  const uint2 *ent;  // per workgroup: lattice, chunk
   int nblocks;
+  int block;
 };
 
 // Boundary-link kernel entries (velocity map BCs).
@@ -107,6 +107,17 @@ struct InitArgs {
 
 // per: periodic lattice of interior size nx x ny x nz (after an odd step the
 // populations of a face cell sit in the wrapped neighbour's slots).
+// Populations from a host view (Q x len, f_i of each cell) into the layout a
+// step of odd parity reads (slot (x, inv(i))), with rho and u of each cell.
+struct WriteFArgs {
+  char *f;
+  long long qs_bytes;
+  int len;
+  int compressible;
+  const void *in;  // Q * len macroscopic type
+  void *rho, *u;
+};
+
 struct UnpermuteArgs {
   const char *f;
   long long qs_bytes;
@@ -144,6 +155,7 @@ hipError_t launch_vlinks(int q, int dtype, const VLinkArgs &a, hipStream_t s);
 hipError_t launch_closure_features(int dtype, bool odd, const StepArgs *args, long long ncell, float *out,
                                   hipStream_t s);
 hipError_t launch_init(int q, int dtype, const InitArgs &a, hipStream_t s);
+hipError_t launch_write_f(int q, int dtype, const WriteFArgs &a, hipStream_t s);
 hipError_t launch_unpermute(int q, int dtype, const UnpermuteArgs &a, hipStream_t s);
 // partial must hold 5 * reduce_blocks(len) doubles; out: 5 doubles (device).
 int reduce_blocks(int len);

@@ -965,6 +965,45 @@ int aslhip_init_equilibrium_field(aslhip_lattice *h, const void *rho, const void
   return 0;
 }
 
+int aslhip_template(int q, int *c, double *w, int *inv) {
+  if (q != 15 && q != 19 && q != 27) return fail(ASLHIP_EINVAL, "ASL ERROR: no template with %d directions", q);
+  const Templ t = make_templ(q);
+  for (int i = 0; i < q; ++i) {
+    for (int d = 0; d < 3; ++d)
+      if (c) c[3 * i + d] = t.c[i][d];
+    if (w) w[i] = t.w[i];
+    if (inv) inv[i] = t.inv[i];
+  }
+  return 0;
+}
+
+int aslhip_write_f(aslhip_lattice *h, const void *f) {
+  if (int r = check(h)) return r;
+  if (!f) return fail(ASLHIP_EINVAL, "ASL ERROR: write_f needs populations");
+  if (int r = flush_deferred(h)) return r;
+  void *d = nullptr;
+  const size_t bytes = (size_t)h->len * h->Q * h->melem;
+  HIPCHK(hipMalloc(&d, bytes));
+  HIPCHK(hipMemcpyAsync(d, f, bytes, hipMemcpyHostToDevice, h->stream));
+  WriteFArgs a;
+  a.f = h->f;
+  a.qs_bytes = h->qs_bytes;
+  a.len = h->len;
+  a.compressible = h->compressible ? 1 : 0;
+  a.in = d;
+  a.rho = h->rho;
+  a.u = h->u;
+  HIPCHK(launch_write_f(h->Q, h->dtype, a, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipFree(d));
+  h->initialized = true;
+  h->fresh = true;
+  h->bc_since_init = false;
+  h->next_odd = true;
+  h->pending.clear();
+  return 0;
+}
+
 int aslhip_bc_add_slice(aslhip_lattice *h, int kind, unsigned faces, double p, const double u[3], int *bc_id) {
   if (int r = check(h)) return r;
   if (h->periodic) return fail(ASLHIP_EINVAL, "ASL ERROR: slice BCs need a non-periodic lattice");
@@ -1216,9 +1255,9 @@ struct aslhip_batch {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   StepArgs *desc = nullptr;  // [2][n]: plain steps, then steps writing rho/u
-  unsigned short *blat = nullptr;
-  int *first = nullptr;
+  uint2 *ent = nullptr;      // per workgroup: lattice, chunk
   int nblocks = 0;
+  int block = STEP_BLOCK;    // workgroup size of all lattices of the batch
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // chunks starting with an even / odd step
   std::vector<StepArgs> hdesc;                    // host copy of desc (source of the async upload)
 };
@@ -1228,7 +1267,7 @@ namespace {
 void batch_free(aslhip_batch *b) {
   for (auto &g : b->graph)
     if (g) (void)hipGraphExecDestroy(g);
-  void *ptrs[] = {b->desc, b->blat, b->first};
+  void *ptrs[] = {b->desc, b->ent};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->done) (void)hipEventDestroy(b->done);
@@ -1238,10 +1277,31 @@ void batch_free(aslhip_batch *b) {
 int batch_launch(aslhip_batch *b, bool odd, bool macro) {
   BatchLaunch L;
   L.desc = b->desc + (macro ? b->lat.size() : 0);
-  L.lat = b->blat;
-  L.first = b->first;
+  L.ent = b->ent;
   L.nblocks = b->nblocks;
+  L.block = b->block;
   HIPCHK(launch_step_batch(b->Q, b->dtype, odd, b->periodic, b->wide, L, b->stream));
+  return 0;
+}
+
+// The workgroup table for the batch's workgroup size (the lattices in order).
+int batch_tables(aslhip_batch *b) {
+  std::vector<uint2> ent;
+  for (size_t l = 0; l < b->lat.size(); ++l) {
+    const StepArgs a = step_args(b->lat[l], b->block);
+    for (int c = 0; c < a.nblocks; ++c) ent.push_back(make_uint2(static_cast<unsigned>(l), static_cast<unsigned>(c)));
+  }
+  for (auto &g : b->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (b->ent) HIPCHK(hipFree(b->ent));
+  b->ent = nullptr;
+  HIPCHK(hipMalloc(&b->ent, ent.size() * sizeof(uint2)));
+  HIPCHK(hipMemcpy(b->ent, ent.data(), ent.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  b->nblocks = static_cast<int>(ent.size());
   return 0;
 }
 
@@ -1292,7 +1352,7 @@ int batch_step_impl(aslhip_batch *b, int nsteps, unsigned flags) {
     if (h->nvl) return fail(ASLHIP_EINVAL, "ASL ERROR: velocity map BCs are not supported in batches");
     if (h->has_closure) return fail(ASLHIP_EINVAL, "ASL ERROR: the closure is not supported in batches");
     if (h->next_odd != odd) return fail(ASLHIP_ESTATE, "ASL ERROR: batch lattices are at different step parities");
-    desc[l] = step_args(h, STEP_BLOCK);  // batch launches use the full workgroup
+    desc[l] = step_args(h, b->block);
     desc[n + l] = desc[l];
     if (h->comp_rho) desc[n + l].flags |= STEP_F_RHO;
     if (h->comp_vel) desc[n + l].flags |= STEP_F_VEL;
@@ -1334,32 +1394,57 @@ int aslhip_batch_create(aslhip_batch **out, aslhip_lattice *const *lattices, int
   b->dtype = lattices[0]->dtype;
   b->periodic = lattices[0]->periodic;
   b->wide = is_wide(lattices[0]);
-  std::vector<int> first(n);
-  std::vector<unsigned short> blat;
   for (int l = 0; l < n; ++l) {
     aslhip_lattice *h = lattices[l];
     if (!h || h->Q != b->Q || h->dtype != b->dtype || h->periodic != b->periodic || is_wide(h) != b->wide) {
       delete b;
       return fail(ASLHIP_EINVAL, "ASL ERROR: batch lattices must share template, population type and periodicity");
     }
-    const StepArgs a = step_args(h, STEP_BLOCK);
-    first[l] = static_cast<int>(blat.size());
-    blat.insert(blat.end(), a.nblocks, static_cast<unsigned short>(l));
   }
-  b->nblocks = static_cast<int>(blat.size());
   hipError_t e = hipStreamCreate(&b->stream);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&b->desc, 2 * (size_t)n * sizeof(StepArgs));
-  if (e == hipSuccess) e = hipMalloc(&b->blat, blat.size() * sizeof(unsigned short));
-  if (e == hipSuccess) e = hipMalloc(&b->first, (size_t)n * sizeof(int));
-  if (e == hipSuccess) e = hipMemcpy(b->blat, blat.data(), blat.size() * sizeof(unsigned short), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(b->first, first.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     batch_free(b);
     delete b;
     return fail(ASLHIP_EHIP, "ASL ERROR: batch set-up: %s", hipGetErrorString(e));
   }
+  if (int r = batch_tables(b)) {
+    batch_free(b);
+    delete b;
+    return r;
+  }
   *out = b;
+  return 0;
+}
+
+int aslhip_batch_set_tuning(aslhip_batch *b, int knob, int value) {
+  if (!b) return fail(ASLHIP_EINVAL, "ASL ERROR: null batch handle");
+  if (knob != ASLHIP_TUNE_BLOCK) return fail(ASLHIP_EINVAL, "ASL ERROR: batches tune the workgroup size only");
+  if (value < 64 || value > STEP_BLOCK || value % 64)
+    return fail(ASLHIP_EINVAL, "ASL ERROR: workgroup size %d is not a multiple of 64 in 64..%d", value, STEP_BLOCK);
+  if (value == b->block) return 0;
+  b->block = value;
+  return batch_tables(b);
+}
+
+int aslhip_batch_autotune(aslhip_batch *b, int nsteps, int *best_block) {
+  if (!b) return fail(ASLHIP_EINVAL, "ASL ERROR: null batch handle");
+  if (nsteps < 1) return fail(ASLHIP_EINVAL, "ASL ERROR: autotune needs at least one step per candidate");
+  nsteps += nsteps & 1;
+  double best = 1e300;
+  int bb = b->block;
+  for (int blk : {256, 512, 1024}) {
+    if (int r = aslhip_batch_set_tuning(b, ASLHIP_TUNE_BLOCK, blk)) return r;
+    double ms = 0.;
+    if (int r = aslhip_batch_time_steps(b, nsteps, 0u, &ms)) return r;
+    if (ms < best) {
+      best = ms;
+      bb = blk;
+    }
+  }
+  if (int r = aslhip_batch_set_tuning(b, ASLHIP_TUNE_BLOCK, bb)) return r;
+  if (best_block) *best_block = bb;
   return 0;
 }
 

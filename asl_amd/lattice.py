@@ -175,6 +175,11 @@ class Lattice:
         call("aslhip_read_f", self.h, out.ctypes.data)
         return out.reshape((self.q,) + self.S)
 
+    def write_f(self, f):
+        """Set the populations (shape (q,) + S, ASL's weight-normalised f_i); rho and u follow."""
+        a = np.ascontiguousarray(np.asarray(f, dtype=self.host_dtype).reshape(self.q * self.len))
+        call("aslhip_write_f", self.h, a.ctypes.data)
+
     def cell_types(self):
         out = np.empty(self.len, dtype=np.uint8)
         call("aslhip_read_cell_types", self.h, out.ctypes.data)
@@ -268,15 +273,32 @@ class Batch:
         call("aslhip_batch_time_steps", self.h, int(nsteps), flags, ctypes.byref(ms))
         return ms.value
 
+    def set_tuning(self, knob, value):
+        call("aslhip_batch_set_tuning", self.h, int(knob), int(value))
+
+    def autotune(self, nsteps=10):
+        """Time the workgroup sizes over nsteps real steps each and keep the fastest."""
+        b = ctypes.c_int(0)
+        call("aslhip_batch_autotune", self.h, int(nsteps), ctypes.byref(b))
+        return b.value
+
 
 # ---------------------------------------------------------------------------
 # ASL-named layer (src/num/aslLBGK.h, src/num/aslLBGKBC.h)
 # ---------------------------------------------------------------------------
 class VectorTemplate:
-    """``asl::VectorTemplate`` handle for d3q15() / d3q19() / d3q27()."""
+    """``asl::VectorTemplate`` for d3q15() / d3q19() / d3q27(): the library's
+    direction vectors c (q x 3), weights w and inverse directions, in ASL's order."""
 
     def __init__(self, q):
         self.q = q
+        c = (ctypes.c_int * (3 * q))()
+        w = (ctypes.c_double * q)()
+        inv = (ctypes.c_int * q)()
+        call("aslhip_template", int(q), c, w, inv)
+        self.c = np.array(c[:], dtype=np.int64).reshape(q, 3)
+        self.w = np.array(w[:])
+        self.inv = np.array(inv[:], dtype=np.int64)
 
 
 def d3q15():

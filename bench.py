@@ -264,10 +264,12 @@ def batch_study(steps, warmup):
     lat = [batch_cavity(n, t) for n, t in specs]
     B = asl_amd.Batch(lat)
     B.step(max(warmup, 1), write_macro=False)
+    block = B.autotune(10)
     ms_batch = B.time_steps(steps, graph=True)
     solo = [batch_cavity(n, t) for n, t in specs]
     for L in solo:
         L.step(max(warmup, 1), write_macro=False)
+        L.autotune(4)
     for L in solo:
         L.synchronize()
     t0 = time.perf_counter()
@@ -286,7 +288,7 @@ def batch_study(steps, warmup):
     ms_interleaved = (time.perf_counter() - t0) * 1e3 * steps / min(steps, 100)
     print(json.dumps({"study": "batched lattices", "lattices": len(specs), "cells": cells,
                       "sizes": [n for n, _ in specs], "steps": steps,
-                      "batch_graph_mlups": round(cells * steps / ms_batch / 1e3, 1),
+                      "batch_graph_mlups": round(cells * steps / ms_batch / 1e3, 1), "batch_block": block,
                       "batch_gbps": round(152.0 * cells * steps / ms_batch / 1e6, 1),
                       "per_lattice_streams_mlups": round(cells * steps / ms_solo / 1e3, 1),
                       "per_lattice_interleaved_mlups": round(cells * steps / ms_interleaved / 1e3, 1)}), flush=True)
@@ -311,19 +313,38 @@ def closure_study(nc, tau_fine, chunk, snaps, eval_chunks):
     ke_f, wall_f = ex.ke_history(F, chunk, eval_chunks, fine=True)
     del F
     log("# fine run done (%.1f s)" % wall_f)
+    ke_ff = ex.filtered_ke_history(chunk, eval_chunks)
+    log("# filtered fine kinetic energy done")
+    # output scale chosen on the training window only (0 = plain LBGK)
+    calib = {}
+    for a in (0.0, 0.25, 0.5, 1.0, 1.5):
+        C = ex.coarse()
+        if a > 0:
+            C.set_closure(params, mu, sc, out * a)
+        ke_a, _ = ex.ke_history(C, chunk, snaps)
+        calib[a] = float(np.abs(ke_a / ke_ff[:snaps + 1] - 1.0).max()) if np.isfinite(ke_a).all() else float("inf")
+    alpha = min(calib, key=calib.get)
+    log("# output scale on the training window: %s -> %s" % (calib, alpha))
     P = ex.coarse()
     ke_p, wall_p = ex.ke_history(P, chunk, eval_chunks)
     C = ex.coarse()
-    C.set_closure(params, mu, sc, out)
+    C.set_closure(params, mu, sc, out * max(alpha, 1e-30))
     ke_c, wall_c = ex.ke_history(C, chunk, eval_chunks)
 
-    def err(ke):
+    def err(ke):  # against the fine run's total kinetic energy, both normalised by their start
         return float(np.abs((ke / ke[0]) / (ke_f / ke_f[0]) - 1.0).max())
+
+    def err_filtered(ke):  # against the box-filtered fine run (the field the coarse lattice represents)
+        return float(np.abs(ke / ke_ff - 1.0).max())
     print(json.dumps({"study": "learned closure, D3Q19 decaying turbulence", "coarse": nc, "fine": ex.nf,
                       "tau_fine": tau_fine, "tau_coarse": 0.5 + 3 * ex.nu_c, "coarse_steps_per_chunk": chunk,
-                      "train_snapshots": snaps, "eval_chunks": eval_chunks,
-                      "ke_fine": ke_f.tolist(), "ke_coarse_plain": ke_p.tolist(), "ke_coarse_closure": ke_c.tolist(),
+                      "train_snapshots": snaps, "eval_chunks": eval_chunks, "output_scale_factor": alpha,
+                      "calibration_errors": {str(k): v for k, v in calib.items()},
+                      "ke_fine": ke_f.tolist(), "ke_fine_filtered": ke_ff.tolist(), "ke_coarse_plain": ke_p.tolist(),
+                      "ke_coarse_closure": ke_c.tolist(),
                       "max_rel_ke_decay_error_plain": err(ke_p), "max_rel_ke_decay_error_closure": err(ke_c),
+                      "max_rel_ke_error_vs_filtered_plain": err_filtered(ke_p),
+                      "max_rel_ke_error_vs_filtered_closure": err_filtered(ke_c),
                       "wall_s_fine": wall_f, "wall_s_coarse_plain": wall_p, "wall_s_coarse_closure": wall_c,
                       "speedup_closure_vs_fine": wall_f / wall_c, "train_s": train_s,
                       "train_loss_first": float(losses[:20].mean()), "train_loss_last": float(losses[-20:].mean())}),

@@ -93,6 +93,12 @@ int aslhip_last_error(char *buf, size_t n);
 /* Library version and the device it runs on (name written to buf). */
 int aslhip_device_info(char *buf, size_t n, int *cu_count, double *hbm_bytes);
 
+/* asl::d3q15() / d3q19() / d3q27() — src/math/aslTemplates.cxx:146-200:
+ * the direction vectors c (q x 3, ASL's order), weights w (q) and inverse
+ * directions inv (q) of the template this library uses for q (any output
+ * may be NULL).  D3Q27 weights are the textbook ones (ASL defines none). */
+int aslhip_template(int q, int *c, double *w, int *inv);
+
 /* asl::LBGK::LBGK(Block, Param nu, const VectorTemplate*, bool compVel,
  * bool compRho, CommandQueue)  — src/num/aslLBGK.h:112-116,
  * src/num/aslLBGK.cxx:114-158.  nx, ny, nz: internal block size; q: 15, 19
@@ -171,6 +177,13 @@ int aslhip_step(aslhip_lattice *h, int nsteps, unsigned flags);
 
 /* Blocks until the lattice's stream is idle. */
 int aslhip_synchronize(aslhip_lattice *h);
+
+/* Writes the populations from a host view in the layout aslhip_read_f
+ * returns (Q * len elements of the host element type, f_i of each cell,
+ * weight-normalised as ASL stores them) and sets rho and u from them: the
+ * write side of asl::LBGK::getF() (src/num/aslLBGK.h:125-127), e.g. to
+ * restore a checkpoint.  Like initF, the lattice then starts afresh. */
+int aslhip_write_f(aslhip_lattice *h, const void *f);
 
 /* asl::LBGK::getRho() / getVelocity() / getF() host views —
  * src/num/aslLBGK.h:125-127.  out: len (rho), 3*len (u) or Q*len (f)
@@ -266,6 +279,10 @@ int aslhip_batch_destroy(aslhip_batch *b);
 int aslhip_batch_step(aslhip_batch *b, int nsteps, unsigned flags);
 /* As aslhip_batch_step, timed with HIP events on the batch's stream. */
 int aslhip_batch_time_steps(aslhip_batch *b, int nsteps, unsigned flags, double *ms_total);
+/* Workgroup size of the batch's launches (ASLHIP_TUNE_BLOCK only), and the
+ * autotuner choosing it (256/512/1024) by timing nsteps real steps each. */
+int aslhip_batch_set_tuning(aslhip_batch *b, int knob, int value);
+int aslhip_batch_autotune(aslhip_batch *b, int nsteps, int *best_block);
 
 #ifdef __cplusplus
 }

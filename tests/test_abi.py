@@ -7,6 +7,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from asl_amd import _lib
@@ -55,3 +56,14 @@ def test_oracle_is_not_linked_into_the_product():
     # the product library must not depend on the oracle
     data = open(_lib.LIB_PATH, "rb").read()
     assert b"orc_" not in data and b"lbgk_oracle" not in data
+
+
+def test_template_entry_point_matches_oracle_templates():
+    """aslhip_template (a host-only call) reports the same tables the oracle is pinned on."""
+    import asl_amd
+    from oracle import template
+    for q in (15, 19, 27):
+        t = asl_amd.VectorTemplate(q)
+        c, w, inv = template(q)
+        assert np.array_equal(t.c, c) and np.array_equal(t.inv, inv)
+        np.testing.assert_allclose(t.w, w, rtol=0, atol=1e-16)

@@ -78,3 +78,17 @@ def test_batch_member_matches_oracle():
     G = members[-1]
     assert np.array_equal(interior(G.rho()), interior(O.rho()).astype(np.float32))
     assert np.array_equal(interior(G.velocity()), interior(O.velocity()).astype(np.float32))
+
+
+def test_batch_workgroup_size_keeps_results():
+    solo = [cavity(n, t) for n, t in SPECS]
+    for L in solo:
+        L.step(130)
+    members = [cavity(n, t) for n, t in SPECS]
+    B = asl_amd.Batch(members)
+    B.set_tuning(_lib.TUNE_BLOCK, 256)
+    B.step(60, graph=True)
+    assert B.autotune(5) in (256, 512, 1024)  # 3 x 6 real steps
+    B.step(130 - 60 - 18, graph=True)
+    for L, R in zip(members, solo):
+        assert np.array_equal(interior(L.f()), interior(R.f()))
