@@ -22,6 +22,7 @@ CODE_FLUID, CODE_FLUID_BC, CODE_SKIP, CODE_GHOST = 0, 1, 2, 3
 TUNE_XCD_REMAP = 1
 TUNE_BLOCK = 2
 CLOSURE_NPARAM = 1345
+CLOSURE_FP32, CLOSURE_BF16_MFMA = 0, 1
 
 # Every entry point of include/aslhip.h with its ctypes signature.
 _vp = ctypes.c_void_p
@@ -65,6 +66,7 @@ SIGNATURES = {
     "aslhip_batch_autotune": [_vp, ctypes.c_int, _ip],
     "aslhip_closure_set": [_vp, _fp, _fp, _fp, ctypes.c_float],
     "aslhip_closure_features": [_vp, _fp],
+    "aslhip_closure_set_mode": [_vp, ctypes.c_int],
     "aslhip_mlp_create": [ctypes.POINTER(_vp), _fp],
     "aslhip_mlp_destroy": [_vp],
     "aslhip_mlp_set_data": [_vp, _fp, _fp, ctypes.c_longlong],

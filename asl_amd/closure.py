@@ -152,12 +152,13 @@ def eddy_viscosity(u_fine, r=4, floor=-np.inf):
 class Experiment:
     """Fine / coarse decaying-turbulence runs sharing one initial field."""
 
-    def __init__(self, n_coarse=128, tau_fine=0.55, u_rms=0.03, seed=20261202, r=4):
+    def __init__(self, n_coarse=128, tau_fine=0.55, u_rms=0.03, seed=20261202, r=4, ce_init=False):
         self.nc, self.r = n_coarse, r
         self.nf = n_coarse * r
         self.nu_f = (tau_fine - 0.5) / 3.0
         self.nu_c = self.nu_f / r
         self.seed, self.u_rms = seed, u_rms
+        self.ce_init = ce_init  # start from equilibrium + first Chapman-Enskog term (else equilibrium)
         self.u0 = turbulence_field(self.nf, seed, u_rms)
         # the pressure that goes with the field, so neither lattice starts with an acoustic transient
         self.rho0 = np.float32(1.0) + np.float32(3.0) * pressure_field(self.u0)
@@ -166,12 +167,18 @@ class Experiment:
         L = Lattice((self.nc,) * 3, 19, dtype="f32", periodic=True, nu=self.nu_c)
         ub = box_filter(self.u0) if u is None else u
         rb = box_filter(self.rho0) if rho is None else rho
-        L.write_f(with_ghosts(populations(rb, ub, 3.0 * self.nu_c + 0.5)))
+        if self.ce_init:
+            L.write_f(with_ghosts(populations(rb, ub, 3.0 * self.nu_c + 0.5)))
+        else:
+            L.init_equilibrium(with_ghosts(rb.astype(np.float32)), with_ghosts(ub.astype(np.float32)))
         return L
 
     def fine(self):
         L = Lattice((self.nf,) * 3, 19, dtype="f32", periodic=True, nu=self.nu_f)
-        L.write_f(with_ghosts(populations(self.rho0, self.u0, 3.0 * self.nu_f + 0.5)))
+        if self.ce_init:
+            L.write_f(with_ghosts(populations(self.rho0, self.u0, 3.0 * self.nu_f + 0.5)))
+        else:
+            L.init_equilibrium(with_ghosts(self.rho0), with_ghosts(self.u0))
         return L
 
     def training_data(self, coarse_steps_between, snapshots, log=print):

@@ -51,6 +51,96 @@ __device__ __forceinline__ void closure_features(const float (&f)[Q], float r, c
   x[6] = r - 1.f;
 }
 
+__device__ __forceinline__ float closure_relax(float y, const ClosureParams &c, float tau0) {
+  const float tau = tau0 + y * c.out;
+  const float lo = 0.5f + 0.5f * (tau0 - 0.5f);
+  return 1.f / (tau < lo ? lo : tau);
+}
+
+// Round to the nearest BF16 (ties to even); the bits of the result.
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+  unsigned u = __float_as_uint(x);
+  if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<unsigned short>(u >> 16);
+}
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+// The network on the matrix cores: every lane of the wavefront brings its own
+// cell's inputs, all 64 lanes must be active.  Two column blocks of 32 cells
+// (cells 0-31, 32-63 of the wavefront) go through
+//   H1^T = relu(W1' xn')   one 32x32x8 BF16 MFMA (xn' = (xn, 1), W1' = (W1, b1)),
+//   H2^T = relu(W2 H1^T + b2)   four 32x32x8 BF16 MFMAs, FP32 accumulation,
+//   y = w3 . h2 + b3       FP32: each lane sums the 16 hidden units it holds,
+//                          the two halves of the wavefront are added.
+// v_mfma_f32_32x32x8bf16_1k operand layout (lane l): A[l % 32][4 (l / 32) + j],
+// B[4 (l / 32) + j][l % 32], j < 4; result register r holds
+// D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32] — the layout the next layer's B
+// operand needs for k-block r / 4, so layers chain without shuffles.
+__device__ __forceinline__ float closure_y_mfma(const float (&x)[7], const ClosureParams &c) {
+  const int l = static_cast<int>(threadIdx.x) & 63;
+  const int col = l & 31, half = l >> 5;
+  unsigned short own[8];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) own[j] = bf16_bits((x[j] - c.mu[j]) * c.sc[j]);
+  own[7] = 0x3f80;  // 1.0: the bias column of W1'
+  // inputs 4 half .. 4 half + 3 of the partner lane's cell (lane l ^ 32)
+  unsigned short par[4];
+  {
+    const unsigned lo = static_cast<unsigned>(own[4 * (1 - half)]) | (static_cast<unsigned>(own[4 * (1 - half) + 1]) << 16);
+    const unsigned hi = static_cast<unsigned>(own[4 * (1 - half) + 2]) | (static_cast<unsigned>(own[4 * (1 - half) + 3]) << 16);
+    const unsigned plo = static_cast<unsigned>(__shfl_xor(static_cast<int>(lo), 32));
+    const unsigned phi = static_cast<unsigned>(__shfl_xor(static_cast<int>(hi), 32));
+    par[0] = static_cast<unsigned short>(plo & 0xffffu);
+    par[1] = static_cast<unsigned short>(plo >> 16);
+    par[2] = static_cast<unsigned short>(phi & 0xffffu);
+    par[3] = static_cast<unsigned short>(phi >> 16);
+  }
+  // weights (BF16-rounded floats, so the top halves are exact)
+  bf16x4_t a1;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = 4 * half + j;
+    const float wv = k < 7 ? c.p[CL_W1 + col * 7 + k] : c.p[CL_B1 + col];
+    a1[j] = static_cast<short>(bf16_bits(wv));
+  }
+  float y[2];
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb) {
+    // column block cb holds cells 32 cb + col; this lane supplies inputs 4 half .. 4 half + 3
+    bf16x4_t b1;
+    const bool mine = (half == cb);  // the cell 32 cb + col is this lane's own
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b1[j] = static_cast<short>(mine ? own[4 * half + j] : par[j]);
+    f32x16_t d1 = {};
+    d1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, b1, d1, 0, 0, 0);
+    f32x16_t d2;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) d2[r] = c.p[CL_B2 + 8 * (r / 4) + 4 * half + (r % 4)];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      bf16x4_t a2, b2;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a2[j] = static_cast<short>(bf16_bits(c.p[CL_W2 + col * CL_H + 8 * kb + 4 * half + j]));
+        const float h = d1[4 * kb + j];
+        b2[j] = static_cast<short>(bf16_bits(h > 0.f ? h : 0.f));
+      }
+      d2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, b2, d2, 0, 0, 0);
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const float h = d2[r] > 0.f ? d2[r] : 0.f;
+      part = __builtin_fmaf(c.p[CL_W3 + 8 * (r / 4) + 4 * half + (r % 4)], h, part);
+    }
+    const float other = __shfl_xor(part, 32);
+    y[cb] = (half == 0 ? part + other : other + part) + c.p[CL_B3];
+  }
+  return y[half];
+}
+
 // tau0: the plain relaxation time 3 nu + 1/2 (float).
 __device__ __forceinline__ float closure_w(const float (&x)[7], const ClosureParams &c, float tau0) {
   float xn[7], h1[CL_H];
@@ -72,9 +162,7 @@ __device__ __forceinline__ float closure_w(const float (&x)[7], const ClosurePar
     const float h2 = a > 0.f ? a : 0.f;
     y = __builtin_fmaf(c.p[CL_W3 + o], h2, y);
   }
-  const float tau = tau0 + y * c.out;
-  const float lo = 0.5f + 0.5f * (tau0 - 0.5f);
-  return 1.f / (tau < lo ? lo : tau);
+  return closure_relax(y, c, tau0);
 }
 
 }  // namespace aslhip

@@ -146,13 +146,14 @@ __device__ __forceinline__ void moments(const C (&v)[Q], C &r, C (&m)[3]) {
 // replaces w per cell (FP32 collision only).
 template <int Q, class C, bool CL = false>
 __device__ __forceinline__ void collide(C (&v)[Q], C w, bool compressible, bool omega, const C (&om)[3], C &r_o,
-                                        C (&m_o)[3], const ClosureParams *cl = nullptr, float tau0 = 0.f) {
+                                        C (&m_o)[3], const ClosureParams *cl = nullptr, float tau0 = 0.f,
+                                        bool cl_mfma = false) {
   C m[3], r;
   moments<Q, C>(v, r, m);
   if constexpr (CL) {  // the learned closure sets the cell's relaxation
     float x[7];
     closure_features<Q>(v, r, m, x);
-    w = closure_w(x, *cl, tau0);
+    w = cl_mfma ? closure_relax(closure_y_mfma(x, *cl), *cl, tau0) : closure_w(x, *cl, tau0);
   }
   // LBGK::setOmega (src/num/aslLBGK.cxx:207-208): momentum += (2/w) omega x
   // momentum, one component after the other, each seeing the ones before it
@@ -226,12 +227,25 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L) {
   constexpr unsigned ES = sizeof(S);
   const int plane = L / a.bpp;
   const int chunk = L - plane * a.bpp;
-  const unsigned off = static_cast<unsigned>(chunk * a.block) + threadIdx.x;
-  if (off >= static_cast<unsigned>(a.ny * a.sy)) return;
+  unsigned off = static_cast<unsigned>(chunk * a.block) + threadIdx.x;
+  // The closure variant keeps every lane to the end (its network may run on
+  // matrix instructions across the wavefront): a lane without a cell of its
+  // own works on a stand-in interior cell of its plane and stores nothing.
+  bool act = off < static_cast<unsigned>(a.ny * a.sy);
+  if constexpr (!CL) {
+    if (!act) return;
+  } else {
+    if (!act) off = 1u;
+  }
   // Row and column inside the plane (exact: off * sy < 2^40, see StepArgs::zmagic).
   const unsigned row = static_cast<unsigned>((static_cast<unsigned long long>(off) * a.zmagic) >> 40);
-  const unsigned z = off - row * static_cast<unsigned>(a.sy);
-  if (z == 0 || z == static_cast<unsigned>(a.sy - 1)) return;  // ghost columns
+  unsigned z = off - row * static_cast<unsigned>(a.sy);
+  if (z == 0 || z == static_cast<unsigned>(a.sy - 1)) {  // ghost columns
+    if constexpr (!CL) return;
+    act = false;
+    off = row * static_cast<unsigned>(a.sy) + 1u;
+    z = 1u;
+  }
   const unsigned idx = static_cast<unsigned>((plane + 1) * a.sx + a.sy) + off;
   const int x = plane + 1, y = static_cast<int>(row) + 1, zi = static_cast<int>(z);
   const unsigned ib = idx * ES;
@@ -353,9 +367,10 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) om[d] = sizeof(C) == 4 ? C(a.om32[d]) : C(a.om64[d]);
   }
-  collide<Q, C, CL>(v, C(a.w), (a.flags & STEP_F_COMPRESSIBLE) != 0, omega, om, r, m, a.cl, a.cl_tau0);
+  collide<Q, C, CL>(v, C(a.w), (a.flags & STEP_F_COMPRESSIBLE) != 0, omega, om, r, m, a.cl, a.cl_tau0,
+                    (a.flags & STEP_F_CLOSURE_MFMA) != 0);
 
-  if (!live) return;
+  if (!live || !act) return;
   // Stores address every direction from fd[0] too (one scalar base pair
   // instead of Q live pointers) unless the lattice is too large for it.
   auto st = [&](int k) {

@@ -18,7 +18,8 @@ enum : unsigned {
   STEP_F_COMPRESSIBLE = 8u,  // compressible equilibrium (ASL default)
   STEP_F_OMEGA = 16u,        // rotating frame: momentum += (2/w) omega x momentum
   STEP_F_CLOSURE = 32u,      // per-cell relaxation time from the closure network
-  STEP_F_COUNT = 64u,        // number of flag combinations
+  STEP_F_CLOSURE_MFMA = 64u, // ... evaluated with BF16 matrix instructions
+  STEP_F_COUNT = 128u,       // number of flag combinations
 };
 
 // One fused collide+stream pass (AA pattern) over the ghost-inclusive block.

@@ -142,6 +142,7 @@ struct aslhip_lattice {
   double omega[3] = {0., 0., 0.};
   // learned closure (network parameters in device memory)
   bool has_closure = false;
+  bool closure_mfma = true;  // evaluate the network with BF16 matrix instructions
   ClosureParams *d_cl = nullptr;
   float *d_feat = nullptr;  // closure features scratch (7 per interior cell)
   double dx = 1., pos0[3] = {0., 0., 0.};
@@ -200,7 +201,7 @@ void free_all(aslhip_lattice *h) {
 // StepArgs::flags bits fixed by the lattice's parameters.
 unsigned base_flags(const aslhip_lattice *h) {
   return (h->compressible ? STEP_F_COMPRESSIBLE : 0u) | (h->has_omega ? STEP_F_OMEGA : 0u) |
-         (h->has_closure ? STEP_F_CLOSURE : 0u);
+         (h->has_closure ? STEP_F_CLOSURE : 0u) | (h->has_closure && h->closure_mfma ? STEP_F_CLOSURE_MFMA : 0u);
 }
 
 double relaxation(const aslhip_lattice *h) {
@@ -885,6 +886,15 @@ int aslhip_closure_set(aslhip_lattice *h, const float *params, const float mu[7]
   HIPCHK(hipStreamSynchronize(h->stream));  // steps queued earlier read the old parameters
   HIPCHK(hipMemcpy(h->d_cl, &c, sizeof c, hipMemcpyHostToDevice));
   h->has_closure = true;
+  return 0;
+}
+
+int aslhip_closure_set_mode(aslhip_lattice *h, int mode) {
+  if (int r = check(h)) return r;
+  if (mode != ASLHIP_CLOSURE_FP32 && mode != ASLHIP_CLOSURE_BF16_MFMA)
+    return fail(ASLHIP_EINVAL, "ASL ERROR: unknown closure mode %d", mode);
+  if (int r = flush_deferred(h)) return r;
+  h->closure_mfma = mode == ASLHIP_CLOSURE_BF16_MFMA;
   return 0;
 }
 

@@ -80,6 +80,10 @@ class Lattice:
             raise ValueError("closure: 1345 parameters and 7-vectors mu, sc expected")
         call("aslhip_closure_set", self.h, *[a.ctypes.data_as(fp) for a in arrs], float(out_scale))
 
+    def set_closure_mode(self, mode):
+        """CLOSURE_BF16_MFMA (default, matrix cores) or CLOSURE_FP32 (vector ALUs, reproducible)."""
+        call("aslhip_closure_set_mode", self.h, int(mode))
+
     def closure_features(self):
         """Network inputs (float32, shape size + (7,)) of every interior cell for the next step."""
         out = np.empty(tuple(self.size) + (7,), dtype=np.float32)

@@ -244,6 +244,15 @@ int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags,
 #define ASLHIP_CLOSURE_NPARAM 1345
 int aslhip_closure_set(aslhip_lattice *h, const float *params, const float mu[7], const float sc[7],
                        float out_scale);
+/* How the step kernel evaluates the network: ASLHIP_CLOSURE_BF16_MFMA
+ * (default) — BF16 inputs and hidden activations on the matrix cores
+ * (v_mfma_f32_32x32x8bf16_1k, FP32 accumulation), the output layer in FP32;
+ * ASLHIP_CLOSURE_FP32 — every product and sum in FP32 on the vector ALUs, in
+ * a fixed order (reproducible bit for bit on the CPU). */
+#define ASLHIP_CLOSURE_FP32 0
+#define ASLHIP_CLOSURE_BF16_MFMA 1
+int aslhip_closure_set_mode(aslhip_lattice *h, int mode);
+
 /* The network inputs x of every interior cell for the next step (periodic
  * D3Q19 lattices): out holds 7 floats per cell, cells in ASL order. */
 int aslhip_closure_features(aslhip_lattice *h, float *out);

@@ -37,6 +37,7 @@ def load_library():
     lib.orc_set_closure.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]
     lib.orc_set_closure.restype = ctypes.c_int
     lib.orc_closure_features.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.orc_set_closure_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_closure_features.restype = ctypes.c_int
     lib.orc_set_compute_macro.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     lib.orc_set_geometry.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
@@ -129,6 +130,10 @@ class OracleLattice:
                                           self._cl[2].ctypes.data, float(out_scale))
         if rc:
             raise ValueError("the closure needs an FP32 D3Q19 lattice")
+
+    def set_closure_mode(self, mode):
+        """0: FP32 evaluation (bit-identical to the GPU's vector path); 1: BF16 (the matrix-core path)."""
+        self.lib.orc_set_closure_mode(self.h, int(mode))
 
     def closure_features(self):
         """Features (7 per interior cell, float32, shape size + (7,)) of a periodic D3Q19 lattice."""

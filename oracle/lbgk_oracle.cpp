@@ -161,6 +161,7 @@ struct Lattice {
   bool has_closure = false;
   std::vector<float> cl;          // CL_NPARAM values, layout of closure_eval
   float cl_mu[7], cl_sc[7], cl_out = 0.f;
+  int cl_mode = 1;  // 0: FP32 chain; 1: BF16 inputs / hidden activations, exact sums
   double dx = 1., pos0[3] = {0., 0., 0.};
   std::vector<BC<T>> bcs;
   // per-direction constants as ASL prints them (double -> T)
@@ -252,7 +253,41 @@ struct Lattice {
   // The network: h1 = relu(W1 xn + b1), h2 = relu(W2 h1 + b2), y = w3.h2 + b3
   // with xn = (x - mu) * sc; the relaxation time becomes tau0 + y * out, at
   // least 1/2 + (tau0 - 1/2) / 2.
+  // The BF16 evaluation the GPU runs on its matrix cores: inputs and first
+  // hidden layer rounded to BF16, products exact, each unit's sum rounded once
+  // to float (the hardware may round in its own order: agreement to a few
+  // ulp, not bit for bit); the output layer in float in the GPU's order (the
+  // 16 units of each wavefront half summed by fma, then the two halves added).
+  inline T closure_w_bf16(const T *x, T tau0) const {
+    const float *W1 = &cl[0], *b1 = &cl[224], *W2 = &cl[256], *b2 = &cl[1280], *w3 = &cl[1312];
+    float xb[7], h1[32], h2[32];
+    for (int j = 0; j < 7; ++j) xb[j] = bf16_round((float(x[j]) - cl_mu[j]) * cl_sc[j]);
+    for (int o = 0; o < 32; ++o) {
+      double a = double(b1[o]);
+      for (int j = 0; j < 7; ++j) a += double(W1[o * 7 + j]) * double(xb[j]);
+      const float af = float(a);
+      h1[o] = bf16_round(af > 0.f ? af : 0.f);
+    }
+    for (int o = 0; o < 32; ++o) {
+      double a = double(b2[o]);
+      for (int j = 0; j < 32; ++j) a += double(W2[o * 32 + j]) * double(h1[j]);
+      const float af = float(a);
+      h2[o] = af > 0.f ? af : 0.f;
+    }
+    float p[2] = {0.f, 0.f};
+    for (int half = 0; half < 2; ++half)
+      for (int r = 0; r < 16; ++r) {
+        const int u = 8 * (r / 4) + 4 * half + r % 4;
+        p[half] = std::fma(w3[u], h2[u], p[half]);
+      }
+    const float y = (p[0] + p[1]) + cl[1344];
+    const float tau = float(tau0) + y * cl_out;
+    const float lo = 0.5f + 0.5f * (float(tau0) - 0.5f);
+    return T(1.) / T(tau < lo ? lo : tau);
+  }
+
   inline T closure_w(const T *x, T tau0) const {
+    if (cl_mode == 1) return closure_w_bf16(x, tau0);
     const float *W1 = &cl[0], *b1 = &cl[224], *W2 = &cl[256], *b2 = &cl[1280], *w3 = &cl[1312];
     float xn[7], h1[32], h2[32];
     for (int j = 0; j < 7; ++j) xn[j] = (float(x[j]) - cl_mu[j]) * cl_sc[j];
@@ -673,6 +708,10 @@ int orc_set_closure(void *h, const float *params, const float *mu, const float *
     L.cl_out = out;
   });
   return rc;
+}
+
+void orc_set_closure_mode(void *h, int mode) {
+  dispatch(h, [&](auto &L) { L.cl_mode = mode; });
 }
 
 // Closure features of every interior cell (periodic lattices): the inputs the

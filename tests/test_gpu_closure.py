@@ -48,6 +48,8 @@ def test_closure_step_periodic_bitwise():
     args = closure_args()
     G.set_closure(*args)
     O.set_closure(*args)
+    G.set_closure_mode(_lib.CLOSURE_FP32)
+    O.set_closure_mode(_lib.CLOSURE_FP32)
     G.step(25)
     O.run(25)
     s = (slice(None),) + (slice(1, -1),) * 3
@@ -68,10 +70,37 @@ def test_closure_step_cavity_bitwise():
     args = closure_args(11)
     G.set_closure(*args)
     O.set_closure(*args)
+    G.set_closure_mode(_lib.CLOSURE_FP32)
+    O.set_closure_mode(_lib.CLOSURE_FP32)
     bcs = [add_bc(G, O, _lib.BC_CONSTANT_VELOCITY, _lib.ZE, u=(0.08, 0, 0)), add_bc(G, O, _lib.BC_NOSLIP, WALLS)]
     run(G, O, 120, bcs)
     d = compare(G, O)
     assert d["rho"] == 0.0 and d["u"] == 0.0 and d["f"] == 0.0, d
+
+
+@pytest.mark.parametrize("size", [(10, 9, 8), (33, 17, 12)])
+def test_closure_matrix_cores_match_oracle(size):
+    """The BF16 matrix-core evaluation (the default) against the oracle's BF16
+    model: same roundings, sums in a different order, so a few ulp apart."""
+    G, O = periodic_pair(size)
+    args = closure_args(5)
+    G.set_closure(*args)
+    O.set_closure(*args)  # both default to the BF16 evaluation
+    s = (slice(None),) + (slice(1, -1),) * 3
+    G.step(1)
+    O.run(1)
+    d1 = np.abs(G.f()[s].astype(np.float64) - O.f()[s]).max()
+    G.step(20)
+    O.run(20)
+    d = np.abs(G.f()[s].astype(np.float64) - O.f()[s]).max()
+    print("closure BF16 MFMA vs oracle: max|df| after 1 step %.2e, after 21 steps %.2e" % (d1, d))
+    assert d1 < 2e-6 and d < 2e-5
+    # the FP32 evaluation is a different rounding of the same network
+    G2, _ = periodic_pair(size)
+    G2.set_closure(*args)
+    G2.set_closure_mode(_lib.CLOSURE_FP32)
+    G2.step(21)
+    assert 0 < np.abs(G2.f()[s].astype(np.float64) - G.f()[s]).max() < 1e-3
 
 
 def test_closure_rejects_other_lattices():

@@ -103,10 +103,11 @@ def random_state():
     return size, f0
 
 
-def run_oracle(size, f0, nu, closure=None):
+def run_oracle(size, f0, nu, closure=None, mode=0):
     L = OracleLattice(size, 19, fp64=False, nu=nu, periodic=True)
     if closure is not None:
         L.set_closure(*closure)
+        L.set_closure_mode(mode)
     L.write_f(f0)
     L.execute()
     return L.f()[:, 1:-1, 1:-1, 1:-1]
@@ -124,6 +125,38 @@ def test_closure_step_matches_numpy(random_state):
     ref, tau = closure_step_numpy(f0[:, 1:-1, 1:-1, 1:-1].astype(np.float64), c, w, nu, p, mu, sc, 0.05)
     assert (tau > 3 * nu + 0.5 + 1e-4).mean() > 0.3  # the network does act
     assert np.abs(got - ref).max() < 3e-6
+
+
+def bf16_model_tau(x, params, mu, sc, out, tau0):
+    """The matrix-core evaluation in numpy: BF16 inputs and first hidden layer, exact sums."""
+    W1, b1, W2, b2, w3, b3 = mlp.unpack(mlp.bf16_round(params).astype(np.float64))
+    xb = mlp.bf16_round(((x.astype(np.float32) - mu) * sc).astype(np.float32)).astype(np.float64)
+    h1 = mlp.bf16_round(np.maximum(xb @ W1.T + b1, 0).astype(np.float32)).astype(np.float64)
+    h2 = np.maximum(h1 @ W2.T + b2, 0)
+    y = h2 @ w3 + b3
+    return np.maximum(tau0 + y * out, 0.5 + 0.5 * (tau0 - 0.5))
+
+
+def test_bf16_evaluation_matches_numpy_model(random_state):
+    size, f0 = random_state
+    c, w, _ = template(19)
+    nu = 0.01
+    p = mlp.init_params(7)
+    p[mlp.NPARAM - 1] = 0.3
+    mu, sc, out = np.zeros(7, np.float32), np.full(7, 40.0, np.float32), 0.05
+    got = run_oracle(size, f0, nu, (p, mu, sc, out), mode=1)
+    fp32 = run_oracle(size, f0, nu, (p, mu, sc, out), mode=0)
+    fi = f0[:, 1:-1, 1:-1, 1:-1].astype(np.float64)
+    pulled = np.stack([np.roll(fi[i], tuple(c[i]), axis=(0, 1, 2)) for i in range(19)])
+    x, rho, m = numpy_features(pulled, c, w)
+    tau = bf16_model_tau(x.reshape(-1, 7), p, mu, sc, out, 3 * nu + 0.5).reshape(rho.shape)
+    ref = np.empty_like(pulled)
+    for i in range(19):
+        cm = np.einsum("a,a...->...", c[i], m)
+        feq = rho + 3 * cm + 4.5 * cm * cm / rho - 1.5 * (m * m).sum(0) / rho
+        ref[i] = pulled[i] - (pulled[i] - feq) / tau
+    assert np.abs(got - ref).max() < 3e-6
+    assert np.abs(got - fp32).max() > 1e-8  # the two evaluations do differ
 
 
 def test_zero_correction_is_plain_bgk(random_state):
