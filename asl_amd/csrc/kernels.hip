@@ -413,9 +413,14 @@ __global__ __launch_bounds__(STEP_BLOCK) void k_step(const StepArgs *args) {
 // Many lattices in one launch: block b works on lattice lat[b], chunk
 // b - first[lat[b]] of it; descriptors live in device memory.
 template <int Q, class S, bool ODD, bool PER, bool WIDE>
+// A workgroup takes up to BATCH_RUN consecutive chunks of one lattice
+// (ent.x = lattice | chunks << 16, ent.y = first chunk), so the two dependent
+// loads of its descriptor are paid once per run, not once per chunk.
 __global__ __launch_bounds__(STEP_BLOCK) void k_step_batch(const StepArgs *desc, const uint2 *ent) {
   const uint2 e = ent[blockIdx.x];
-  step_body<Q, S, ODD, PER, WIDE>(desc[e.x], static_cast<int>(e.y));
+  const StepArgs &a = desc[e.x & 0xffffu];
+  const int n = static_cast<int>(e.x >> 16);
+  for (int k = 0; k < n; ++k) step_body<Q, S, ODD, PER, WIDE>(a, static_cast<int>(e.y) + k);
 }
 
 // Closure features of every interior cell of a periodic D3Q19 lattice: the

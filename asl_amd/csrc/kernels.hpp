@@ -9,6 +9,7 @@ namespace aslhip {
 
 // Threads per workgroup of the step kernel (each thread one cell).
 constexpr int STEP_BLOCK = 1024;
+constexpr int BATCH_RUN = 4;  // chunks per workgroup of the batched step kernel
 
 // Step-kernel flags.
 enum : unsigned {
@@ -67,7 +68,7 @@ struct StepArgs {
 // chunk b - first[lat[b]].
 struct BatchLaunch {
   const StepArgs *desc;
-  const uint2 *ent;  // per workgroup: lattice, chunk
+  const uint2 *ent;  // per workgroup: lattice | chunks << 16, first chunk
   int nblocks;
   int block;
 };

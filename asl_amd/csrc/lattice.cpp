@@ -1265,7 +1265,7 @@ struct aslhip_batch {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   StepArgs *desc = nullptr;  // [2][n]: plain steps, then steps writing rho/u
-  uint2 *ent = nullptr;      // per workgroup: lattice, chunk
+  uint2 *ent = nullptr;      // per workgroup: lattice | chunks << 16, first chunk
   int nblocks = 0;
   int block = STEP_BLOCK;    // workgroup size of all lattices of the batch
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // chunks starting with an even / odd step
@@ -1299,7 +1299,10 @@ int batch_tables(aslhip_batch *b) {
   std::vector<uint2> ent;
   for (size_t l = 0; l < b->lat.size(); ++l) {
     const StepArgs a = step_args(b->lat[l], b->block);
-    for (int c = 0; c < a.nblocks; ++c) ent.push_back(make_uint2(static_cast<unsigned>(l), static_cast<unsigned>(c)));
+    for (int c = 0; c < a.nblocks; c += BATCH_RUN) {
+      const int n = std::min(BATCH_RUN, a.nblocks - c);
+      ent.push_back(make_uint2(static_cast<unsigned>(l) | (static_cast<unsigned>(n) << 16), static_cast<unsigned>(c)));
+    }
   }
   for (auto &g : b->graph)
     if (g) {
