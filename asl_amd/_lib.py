@@ -51,6 +51,8 @@ SIGNATURES = {
     "aslhip_read_velocity": [_vp, _vp],
     "aslhip_read_f": [_vp, _vp],
     "aslhip_write_f": [_vp, _vp],
+    "aslhip_surface_force": [_vp, _fp, _vp],
+    "aslhip_surface_flux": [_vp, _fp, _vp],
     "aslhip_template": [ctypes.c_int, _ip, _dp, _ip],
     "aslhip_read_cell_types": [_vp, _vp],
     "aslhip_reduce_conservation": [_vp, _dp],

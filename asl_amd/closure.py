@@ -20,7 +20,7 @@ lattice produced by this same library:
 
 The network trains on the GPU with the library's own forward / backward /
 Adam kernels (``asl_amd.Mlp``).  Plain LBGK stays the fallback
-(``set_closure(None)``) and the judge of the error.  Acoustic scaling: one
+(``set_closure(None)``) and the reference for its error.  Acoustic scaling: one
 coarse step is four fine steps at the same lattice velocity, and the coarse
 viscosity is a quarter of the fine one.
 """

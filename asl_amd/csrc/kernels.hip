@@ -219,8 +219,10 @@ __device__ __forceinline__ double face_val<double>(const StepArgs &a, int f, int
 }
 
 // One cell of one step: chunk L of the lattice described by a.
-template <int Q, class S, bool ODD, bool PER, bool WIDE, bool CL = false>
-__device__ __forceinline__ void step_body(const StepArgs &a, const int L) {
+// PEEK: instead of colliding and storing, write the populations the cell
+// receives (after every BC link is resolved) to peek[k * len + cell].
+template <int Q, class S, bool ODD, bool PER, bool WIDE, bool CL = false, bool PEEK = false>
+__device__ __forceinline__ void step_body(const StepArgs &a, const int L, void *peek = nullptr) {
   using CD = Codec<S>;
   using C = typename CD::C;
   using M = typename CD::M;
@@ -360,6 +362,14 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L) {
     }
   }
 
+  if constexpr (PEEK) {
+    if (!live) return;
+    M *o = static_cast<M *>(peek);
+#pragma unroll
+    for (int k = 0; k < Q; ++k) o[static_cast<long long>(k) * a.len + idx] = M(v[k]);
+    return;
+  }
+
   C r, m[3];
   const bool omega = (a.flags & STEP_F_OMEGA) != 0;
   C om[3] = {C(0), C(0), C(0)};
@@ -410,12 +420,68 @@ __global__ __launch_bounds__(STEP_BLOCK) void k_step(const StepArgs *args) {
   step_body<Q, S, ODD, PER, WIDE, CL>(a, a.remap > 1 ? xcd_remap(b, a.nblocks, a.remap) : b);
 }
 
+// The incoming populations of every computed cell for the next step.
+template <int Q, class S, bool ODD, bool PER>
+__global__ __launch_bounds__(STEP_BLOCK) void k_peek(const StepArgs *args, void *out) {
+  step_body<Q, S, ODD, PER, false, false, true>(*args, static_cast<int>(blockIdx.x), out);
+}
+
+// asl::ComputeSurfaceForceMap / ComputeSurfaceFluxMap (src/num/aslLBGKBC.cxx:
+// 623-736) from two host-layout views: post[i * len + x] = f_i of cell x after
+// the last collision, in_[i * len + x] = f_i that cell x receives next (the
+// value ASL's BCs leave at ghost x - c_i).  For a ghost cell x (map <= 0) next
+// to a computation node y = x + c_i (map > 0), ASL's f_i(x) is in_[i][y];
+// for a computation node x next to a ghost y, ASL's f_inv(i)(y) is
+// in_[inv(i)][x].  Cells with |map| >= 0.9999 get 0 (ASL leaves them as they
+// were); map reads outside the block are -0.9 (a ghost node), as in ASL.
+template <int Q, class S>
+__global__ __launch_bounds__(BLOCK) void k_surface(const SurfaceArgs a) {
+  using C = typename Codec<S>::C;
+  using M = typename Codec<S>::M;
+  const int x = blockIdx.x * BLOCK + threadIdx.x;
+  if (x >= a.len) return;
+  const M *post = static_cast<const M *>(a.post), *in = static_cast<const M *>(a.in);
+  M *out = static_cast<M *>(a.out);
+  const float m0 = a.map[x];
+  C acc[3] = {C(0), C(0), C(0)};
+  if (__builtin_fabsf(m0) < .9999f) {
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const int kb = inv_dir<Q>(i);
+      const long long y = static_cast<long long>(x) + offc<Q>(i, a.sx, a.sy);
+      const float my = (y < 0 || y > a.len - 1) ? -.9f : a.map[y];
+      const bool b0 = m0 <= 0.f && my > 0.f;  // isGhostNode(0) && isComputationNode(i)
+      const bool bi = m0 > 0.f && my <= 0.f;  // isGhostNode(i) && isComputationNode(0)
+      const long long yy = b0 ? y : x;        // the computation node of the link
+      const C f0 = b0 ? C(in[static_cast<long long>(i) * a.len + yy]) : C(post[static_cast<long long>(i) * a.len + x]);
+      const C fI = b0 ? C(post[static_cast<long long>(kb) * a.len + yy]) : C(in[static_cast<long long>(kb) * a.len + x]);
+      if (a.flux) {
+        const C wi = C(Lat<Q>::w(i));
+        acc[0] = acc[0] + (b0 ? (f0 - fI) * wi : C(0));
+        acc[0] = acc[0] + (bi ? (fI - f0) * wi : C(0));
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const C wc = C(Lat<Q>::w(i) * double(Lat<Q>::c[i][d]));
+          acc[d] = acc[d] + (b0 ? ((f0 + fI) - C(2.)) * wc : C(0));
+          acc[d] = acc[d] + (bi ? ((-f0 - fI) + C(2.)) * wc : C(0));
+        }
+      }
+    }
+  }
+  if (a.flux) {
+    out[x] = M(acc[0]);
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[static_cast<long long>(d) * a.len + x] = M(acc[d]);
+  }
+}
+
 // Many lattices in one launch: block b works on lattice lat[b], chunk
 // b - first[lat[b]] of it; descriptors live in device memory.
 template <int Q, class S, bool ODD, bool PER, bool WIDE>
 // A workgroup takes up to BATCH_RUN consecutive chunks of one lattice
-// (ent.x = lattice | chunks << 16, ent.y = first chunk), so the two dependent
-// loads of its descriptor are paid once per run, not once per chunk.
+// (ent.x = lattice | chunks << 16, ent.y = first chunk).
 __global__ __launch_bounds__(STEP_BLOCK) void k_step_batch(const StepArgs *desc, const uint2 *ent) {
   const uint2 e = ent[blockIdx.x];
   const StepArgs &a = desc[e.x & 0xffffu];
@@ -746,6 +812,31 @@ struct InitF {
   }
 };
 template <int Q, class S>
+struct PeekF {
+  static hipError_t run(bool odd, bool per, const StepLaunch &a, void *out, hipStream_t s) {
+    const dim3 grid(a.nblocks), block(a.block);
+    if (per) {
+      if (odd)
+        hipLaunchKernelGGL((k_peek<Q, S, true, true>), grid, block, 0, s, a.args, out);
+      else
+        hipLaunchKernelGGL((k_peek<Q, S, false, true>), grid, block, 0, s, a.args, out);
+    } else {
+      if (odd)
+        hipLaunchKernelGGL((k_peek<Q, S, true, false>), grid, block, 0, s, a.args, out);
+      else
+        hipLaunchKernelGGL((k_peek<Q, S, false, false>), grid, block, 0, s, a.args, out);
+    }
+    return hipGetLastError();
+  }
+};
+template <int Q, class S>
+struct SurfaceF {
+  static hipError_t run(const SurfaceArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL((k_surface<Q, S>), dim3((a.len + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+  }
+};
+template <int Q, class S>
 struct WriteFF {
   static hipError_t run(const WriteFArgs &a, hipStream_t s) {
     hipLaunchKernelGGL((k_write_f<Q, S>), dim3((a.len + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, a);
@@ -787,6 +878,12 @@ hipError_t launch_vlinks(int q, int dtype, const VLinkArgs &a, hipStream_t s) {
   return dispatch<VLinkF>(q, dtype, a, s);
 }
 hipError_t launch_init(int q, int dtype, const InitArgs &a, hipStream_t s) { return dispatch<InitF>(q, dtype, a, s); }
+hipError_t launch_peek(int q, int dtype, bool odd, bool periodic, const StepLaunch &a, void *out, hipStream_t s) {
+  return dispatch<PeekF>(q, dtype, odd, periodic, a, out, s);
+}
+hipError_t launch_surface(int q, int dtype, const SurfaceArgs &a, hipStream_t s) {
+  return dispatch<SurfaceF>(q, dtype, a, s);
+}
 hipError_t launch_write_f(int q, int dtype, const WriteFArgs &a, hipStream_t s) {
   return dispatch<WriteFF>(q, dtype, a, s);
 }

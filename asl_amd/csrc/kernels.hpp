@@ -9,7 +9,10 @@ namespace aslhip {
 
 // Threads per workgroup of the step kernel (each thread one cell).
 constexpr int STEP_BLOCK = 1024;
-constexpr int BATCH_RUN = 4;  // chunks per workgroup of the batched step kernel
+// Chunks per workgroup of the batched step kernel.  Runs of 4 were measured
+// slower on the 64-cavity batch (22.6 against 29.9 GLUPS): fewer workgroups,
+// and a wavefront's chunks do not overlap each other.
+constexpr int BATCH_RUN = 1;
 
 // Step-kernel flags.
 enum : unsigned {
@@ -109,6 +112,16 @@ struct InitArgs {
 
 // per: periodic lattice of interior size nx x ny x nz (after an odd step the
 // populations of a face cell sit in the wrapped neighbour's slots).
+// Surface force (3 x len) or flux (len) of a level-set map, in the
+// macroscopic type, from post-collision and incoming population views.
+struct SurfaceArgs {
+  const float *map;
+  const void *post, *in;  // Q * len each
+  void *out;
+  int sx, sy, len;
+  int flux;               // 1: ComputeSurfaceFluxMap, 0: ComputeSurfaceForceMap
+};
+
 // Populations from a host view (Q x len, f_i of each cell) into the layout a
 // step of odd parity reads (slot (x, inv(i))), with rho and u of each cell.
 struct WriteFArgs {
@@ -158,6 +171,10 @@ hipError_t launch_closure_features(int dtype, bool odd, const StepArgs *args, lo
                                   hipStream_t s);
 hipError_t launch_init(int q, int dtype, const InitArgs &a, hipStream_t s);
 hipError_t launch_write_f(int q, int dtype, const WriteFArgs &a, hipStream_t s);
+// The populations every computed cell receives in the next step (Q * len,
+// macroscopic type, other cells untouched); lattices under 4 GiB.
+hipError_t launch_peek(int q, int dtype, bool odd, bool periodic, const StepLaunch &a, void *out, hipStream_t s);
+hipError_t launch_surface(int q, int dtype, const SurfaceArgs &a, hipStream_t s);
 hipError_t launch_unpermute(int q, int dtype, const UnpermuteArgs &a, hipStream_t s);
 // partial must hold 5 * reduce_blocks(len) doubles; out: 5 doubles (device).
 int reduce_blocks(int len);

@@ -1125,6 +1125,83 @@ int aslhip_read_f(aslhip_lattice *h, void *out) {
   return r;
 }
 
+}  // extern "C"
+
+namespace {
+
+// ComputeSurfaceForceMap / ComputeSurfaceFluxMap on the device: post-collision
+// view, incoming view (peek), then the surface kernel.
+int surface_impl(aslhip_lattice *h, const float *map, void *out, bool flux) {
+  if (int r = check(h)) return r;
+  if (!map || !out) return fail(ASLHIP_EINVAL, "ASL ERROR: surface force/flux needs a map and an output field");
+  if (!h->initialized) return fail(ASLHIP_ESTATE, "ASL ERROR: surface force/flux before initF");
+  if (is_wide(h)) return fail(ASLHIP_EINVAL, "ASL ERROR: surface force/flux on lattices over 4 GiB is not supported");
+  if (int r = flush_deferred(h)) return r;
+  if (int r = prepare(h, false)) return r;
+  if (int r = upload_args(h)) return r;
+  const size_t view = (size_t)h->len * h->Q * h->melem;
+  const size_t outb = (size_t)h->len * (flux ? 1 : 3) * h->melem;
+  void *post = nullptr, *in = nullptr, *dout = nullptr;
+  float *dmap = nullptr;
+  hipError_t e = hipMalloc(&post, view);
+  if (e == hipSuccess) e = hipMalloc(&in, view);
+  if (e == hipSuccess) e = hipMalloc(&dout, outb);
+  if (e == hipSuccess) e = hipMalloc(&dmap, (size_t)h->len * sizeof(float));
+  int r = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(dmap, map, (size_t)h->len * sizeof(float), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(in, 0, view, h->stream);
+  if (e == hipSuccess) {
+    UnpermuteArgs u;
+    u.f = h->f;
+    u.qs_bytes = h->qs_bytes;
+    u.sx = h->sx;
+    u.sy = h->sy;
+    u.len = h->len;
+    u.per = h->periodic ? 1 : 0;
+    u.nx = h->n[0];
+    u.ny = h->n[1];
+    u.nz = h->n[2];
+    u.after_odd = h->next_odd ? 0 : 1;
+    u.out = post;
+    e = launch_unpermute(h->Q, h->dtype, u, h->stream);
+  }
+  if (e == hipSuccess) {
+    StepLaunch a;
+    a.nblocks = h->h_args[0].nblocks;
+    a.block = h->h_args[0].block;
+    a.args = h->d_args + (base_flags(h) & ~(STEP_F_CLOSURE | STEP_F_CLOSURE_MFMA));
+    a.closure = false;
+    e = launch_peek(h->Q, h->dtype, h->next_odd, h->periodic, a, in, h->stream);
+  }
+  if (e == hipSuccess) {
+    SurfaceArgs s;
+    s.map = dmap;
+    s.post = post;
+    s.in = in;
+    s.out = dout;
+    s.sx = h->sx;
+    s.sy = h->sy;
+    s.len = h->len;
+    s.flux = flux ? 1 : 0;
+    e = launch_surface(h->Q, h->dtype, s, h->stream);
+  }
+  if (e == hipSuccess) r = copy_out(h, dout, out, outb);
+  else r = fail(ASLHIP_EHIP, "ASL ERROR: surface force/flux: %s", hipGetErrorString(e));
+  (void)hipStreamSynchronize(h->stream);
+  void *ptrs[] = {post, in, dout, dmap};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aslhip_surface_force(aslhip_lattice *h, const float *map, void *out) { return surface_impl(h, map, out, false); }
+
+int aslhip_surface_flux(aslhip_lattice *h, const float *map, void *out) { return surface_impl(h, map, out, true); }
+
 int aslhip_read_cell_types(aslhip_lattice *h, uint8_t *out) {
   if (int r = check(h)) return r;
   if (h->dirty) {

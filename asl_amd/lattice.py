@@ -179,6 +179,20 @@ class Lattice:
         call("aslhip_read_f", self.h, out.ctypes.data)
         return out.reshape((self.q,) + self.S)
 
+    def surface_force(self, mapv):
+        """ComputeSurfaceForceMap: momentum exchange at the surface of mapv, shape (3,) + S."""
+        m = np.ascontiguousarray(np.asarray(mapv, dtype=np.float32).reshape(self.len))
+        out = np.empty(3 * self.len, dtype=self.host_dtype)
+        call("aslhip_surface_force", self.h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), out.ctypes.data)
+        return out.reshape((3,) + self.S)
+
+    def surface_flux(self, mapv):
+        """ComputeSurfaceFluxMap: mass exchange at the surface of mapv, shape S."""
+        m = np.ascontiguousarray(np.asarray(mapv, dtype=np.float32).reshape(self.len))
+        out = np.empty(self.len, dtype=self.host_dtype)
+        call("aslhip_surface_flux", self.h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), out.ctypes.data)
+        return out.reshape(self.S)
+
     def write_f(self, f):
         """Set the populations (shape (q,) + S, ASL's weight-normalised f_i); rho and u follow."""
         a = np.ascontiguousarray(np.asarray(f, dtype=self.host_dtype).reshape(self.q * self.len))

@@ -178,6 +178,16 @@ int aslhip_step(aslhip_lattice *h, int nsteps, unsigned flags);
 /* Blocks until the lattice's stream is idle. */
 int aslhip_synchronize(aslhip_lattice *h);
 
+/* asl::ComputeSurfaceForceMap / ComputeSurfaceFluxMap execute() —
+ * src/num/aslLBGKBC.cxx:623-736, factories generateComputeSurfaceForce /
+ * Flux (src/num/aslLBGKBC.h:329-336).  Momentum (force, 3 * len elements) or
+ * mass (flux, len) exchanged across the links between the ghost nodes
+ * (map <= 0) and computation nodes (map > 0) of map (len floats, ASL
+ * layout), from the populations after the last step and its BCs; host
+ * arrays of the host element type.  Cells with |map| >= 0.9999 get 0. */
+int aslhip_surface_force(aslhip_lattice *h, const float *map, void *out);
+int aslhip_surface_flux(aslhip_lattice *h, const float *map, void *out);
+
 /* Writes the populations from a host view in the layout aslhip_read_f
  * returns (Q * len elements of the host element type, f_i of each cell,
  * weight-normalised as ASL stores them) and sets rho and u from them: the

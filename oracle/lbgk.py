@@ -38,6 +38,7 @@ def load_library():
     lib.orc_set_closure.restype = ctypes.c_int
     lib.orc_closure_features.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     lib.orc_set_closure_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_surface.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     lib.orc_closure_features.restype = ctypes.c_int
     lib.orc_set_compute_macro.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     lib.orc_set_geometry.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
@@ -130,6 +131,20 @@ class OracleLattice:
                                           self._cl[2].ctypes.data, float(out_scale))
         if rc:
             raise ValueError("the closure needs an FP32 D3Q19 lattice")
+
+    def surface_force(self, mapv):
+        """ComputeSurfaceForceMap on the current state, shape (3,) + S (cells with |map| >= 0.9999: 0)."""
+        m = np.ascontiguousarray(np.asarray(mapv, dtype=np.float32).reshape(self.len))
+        out = np.empty(3 * self.len)
+        self.lib.orc_surface(self.h, m.ctypes.data, 0, out.ctypes.data)
+        return out.reshape((3,) + self.S)
+
+    def surface_flux(self, mapv):
+        """ComputeSurfaceFluxMap on the current state, shape S."""
+        m = np.ascontiguousarray(np.asarray(mapv, dtype=np.float32).reshape(self.len))
+        out = np.empty(self.len)
+        self.lib.orc_surface(self.h, m.ctypes.data, 1, out.ctypes.data)
+        return out.reshape(self.S)
 
     def set_closure_mode(self, mode):
         """0: FP32 evaluation (bit-identical to the GPU's vector path); 1: BF16 (the matrix-core path)."""

@@ -489,6 +489,49 @@ struct Lattice {
     // generateVEOutOfBoundarySafe(a) (src/acl/aclMath/aclMathAlg.cxx:76-81)
     return win(i)[(len + idx) % len];
   }
+  // ComputeSurfaceForceMap / ComputeSurfaceFluxMap::init + execute
+  // (src/num/aslLBGKBC.cxx:623-736) on the pool state after the last step and
+  // its BCs: per cell with |map| < 0.9999, over i = 1..Q-1,
+  //   isBoundary0 = map(x) <= 0 && map(x + c_i) > 0, isBoundaryI = map(x) > 0 && map(x + c_i) <= 0,
+  //   f0 = f_i(x), fI = f_inv(i)(x + c_i) (linear index wrapped, as generateDCFullSafe),
+  //   force += select(0, (f0 + fI - 2) w_i c_i, isBoundary0) + select(0, (-f0 - fI + 2) w_i c_i, isBoundaryI),
+  //   flux += select((f0 - fI) w_i, isBoundary0) + select((fI - f0) w_i, isBoundaryI).
+  // Cells with |map| >= 0.9999 are left to the caller (0 here).
+  void surface(const float *map, bool flux, double *out) {
+    const int Q = vt.Q;
+    std::vector<float> m(map, map + len);
+#pragma omp parallel for schedule(static)
+    for (long j = 0; j < len; ++j) {
+      T acc[3] = {T(0), T(0), T(0)};
+      const float m0 = m[(size_t)j];
+      if (std::fabs(m0) < .9999f) {
+        for (int i = 1; i < Q; ++i) {
+          const long y = j + off(i);
+          const float my = map_safe(m, y, -.9f);
+          const bool b0 = m0 <= 0.f && my > 0.f;
+          const bool bi = m0 > 0.f && my <= 0.f;
+          const T f0 = win(i)[j];
+          const T fI = f_safe(vt.inv[i], y);
+          if (flux) {
+            const T wi = T(vt.w[i]);
+            acc[0] = acc[0] + (b0 ? (f0 - fI) * wi : T(0));
+            acc[0] = acc[0] + (bi ? (fI - f0) * wi : T(0));
+          } else {
+            for (int d = 0; d < 3; ++d) {
+              const T wc = T(vt.w[i] * double(vt.c[i][d]));
+              acc[d] = acc[d] + (b0 ? ((f0 + fI) - T(2.)) * wc : T(0));
+              acc[d] = acc[d] + (bi ? ((-f0 - fI) + T(2.)) * wc : T(0));
+            }
+          }
+        }
+      }
+      if (flux)
+        out[j] = double(acc[0]);
+      else
+        for (int d = 0; d < 3; ++d) out[(size_t)d * len + j] = double(acc[d]);
+    }
+  }
+
   bool comp_node(const BC<T> &bc, long idx) const {
     // isComputationNode (src/num/aslBCond.cxx:204-211;
     // src/math/aslDistanceFunctionAlg.cxx:45-48): map > 0 [&& cdomain > 0]
@@ -708,6 +751,10 @@ int orc_set_closure(void *h, const float *params, const float *mu, const float *
     L.cl_out = out;
   });
   return rc;
+}
+
+void orc_surface(void *h, const float *map, int flux, double *out) {
+  dispatch(h, [&](auto &L) { L.surface(map, flux != 0, out); });
 }
 
 void orc_set_closure_mode(void *h, int mode) {

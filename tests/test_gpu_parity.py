@@ -268,3 +268,25 @@ def test_write_f_round_trip_and_parity(periodic):
     bcs = [] if periodic else [add_bc(G, O, _lib.BC_NOSLIP, 63)]
     run(G, O, 17, bcs)
     assert np.array_equal(G.f()[s], O.f()[s].astype(np.float32))
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_surface_force_and_flux(dtype):
+    """ComputeSurfaceForceMap / FluxMap: GPU (post-collision view + resolved
+    incoming links) against the oracle's pool state, bit for bit on the interior."""
+    size = (20, 14, 12)
+    mapv = geometry.normalize(geometry.sphere(size, 3.0, (8.0, 6.5, 5.5)))
+    G, O = pair(size, 19, nu=0.04, dtype=dtype)
+    G.init_equilibrium(1.0, (0.03, 0, 0))
+    O.init_f(1.0, (0.03, 0, 0))
+    bcs = [add_bc(G, O, _lib.BC_CONSTANT_VELOCITY, _lib.X0, u=(0.03, 0, 0)),
+           add_bc(G, O, _lib.BC_CONSTANT_PRESSURE, _lib.XE, p=1.0),
+           add_bc(G, O, _lib.BC_NOSLIP, _lib.Y0 | _lib.YE | _lib.Z0 | _lib.ZE),
+           add_bc(G, O, _lib.BC_NOSLIP_MAP, mapv=mapv)]
+    run(G, O, 31, bcs)
+    inner = (slice(1, -1),) * 3
+    fg, fo = G.surface_force(mapv), O.surface_force(mapv)
+    qg, qo = G.surface_flux(mapv), O.surface_flux(mapv)
+    assert np.abs(fo[(slice(None),) + inner]).max() > 1e-4
+    assert np.array_equal(fg[(slice(None),) + inner], fo[(slice(None),) + inner].astype(fg.dtype))
+    assert np.array_equal(qg[inner], qo[inner].astype(qg.dtype))
