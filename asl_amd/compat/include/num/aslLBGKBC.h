@@ -103,6 +103,32 @@ class BCMacroFillMap : public NumMethod {
   bool gradient;
 };
 
+// ComputeSurfaceFluxMap / ComputeSurfaceForceMap (src/num/aslLBGKBC.h:247-279):
+// execute() fills the field with the mass / momentum exchanged across the
+// links between the ghost and computation nodes of map (aslhip_surface_flux /
+// aslhip_surface_force).
+class ComputeSurfaceFluxMap : public NumMethod {
+ public:
+  ComputeSurfaceFluxMap(SPLBGK nm, SPDataWithGhostNodesACLData fF, SPAbstractDataWithGhostNodes map);
+  virtual void init() {}
+  virtual void execute();
+
+ protected:
+  SPLBGK num;
+  SPDataWithGhostNodesACLData field;
+  SPAbstractDataWithGhostNodes map;
+};
+
+class ComputeSurfaceForceMap : public ComputeSurfaceFluxMap {
+ public:
+  ComputeSurfaceForceMap(SPLBGK nm, SPDataWithGhostNodesACLData fF, SPAbstractDataWithGhostNodes map)
+      : ComputeSurfaceFluxMap(nm, fF, map) {}
+  virtual void execute();
+};
+
+SPNumMethod generateComputeSurfaceFlux(SPLBGK nm, SPDataWithGhostNodesACLData fF, SPAbstractDataWithGhostNodes map);
+SPNumMethod generateComputeSurfaceForce(SPLBGK nm, SPDataWithGhostNodesACLData fF, SPAbstractDataWithGhostNodes map);
+
 SPBCond generateBCNoSlip(SPLBGK nm, const std::vector<SlicesNames> &sl);
 SPBCond generateBCConstantVelocity(SPLBGK nm, AVec<> v, const std::vector<SlicesNames> &sl);
 SPBCond generateBCConstantPressure(SPLBGK nm, double p, const std::vector<SlicesNames> &sl);

@@ -159,6 +159,50 @@ void BCMacroFillMap::execute() {
   }
 }
 
+ComputeSurfaceFluxMap::ComputeSurfaceFluxMap(SPLBGK nm, SPDataWithGhostNodesACLData fF,
+                                             SPAbstractDataWithGhostNodes m)
+    : num(nm), field(fF), map(m) {}
+
+namespace {
+// Runs the surface kernel and stores its result (ncomp components) in field.
+void surface_into(LBGK &num, AbstractData &field, const std::vector<float> &m, unsigned int ncomp, bool flux) {
+  if (field.getNComponents() != ncomp)
+    errorMessage(flux ? "ComputeSurfaceFluxMap: the flux field needs 1 component"
+                      : "ComputeSurfaceForceMap: the force field needs 3 components");
+  const size_t len = m.size();
+  if (field.getBlock().getSize() != num.getRho()->getBlock().getSize())
+    errorMessage("ComputeSurfaceForceMap: the field's block differs from the lattice's");
+  if (num.typeID() == acl::TYPE_DOUBLE) {
+    std::vector<double> out(ncomp * len);
+    check_abi(flux ? aslhip_surface_flux(num.handle(), m.data(), out.data())
+                   : aslhip_surface_force(num.handle(), m.data(), out.data()));
+    for (unsigned int c = 0; c < ncomp; ++c) std::copy(out.begin() + c * len, out.begin() + (c + 1) * len, field.values_[c].begin());
+  } else {
+    std::vector<float> out(ncomp * len);
+    check_abi(flux ? aslhip_surface_flux(num.handle(), m.data(), out.data())
+                   : aslhip_surface_force(num.handle(), m.data(), out.data()));
+    for (unsigned int c = 0; c < ncomp; ++c)
+      for (size_t i = 0; i < len; ++i) field.values_[c][i] = out[c * len + i];
+  }
+}
+}  // namespace
+
+void ComputeSurfaceFluxMap::execute() {
+  surface_into(*num, *field, map_array(*num, map, "ComputeSurfaceFluxMap"), 1, true);
+}
+
+void ComputeSurfaceForceMap::execute() {
+  surface_into(*num, *field, map_array(*num, map, "ComputeSurfaceForceMap"), 3, false);
+}
+
+SPNumMethod generateComputeSurfaceFlux(SPLBGK nm, SPDataWithGhostNodesACLData fF, SPAbstractDataWithGhostNodes map) {
+  return std::make_shared<ComputeSurfaceFluxMap>(nm, fF, map);
+}
+
+SPNumMethod generateComputeSurfaceForce(SPLBGK nm, SPDataWithGhostNodesACLData fF, SPAbstractDataWithGhostNodes map) {
+  return std::make_shared<ComputeSurfaceForceMap>(nm, fF, map);
+}
+
 SPBCond generateBCNoSlip(SPLBGK nm, const std::vector<SlicesNames> &sl) {
   auto bc = std::make_shared<BCNoSlip>(nm);
   addSlices(*bc, sl);
