@@ -35,8 +35,8 @@ class LBGK : public SingleKernelNM {
   virtual void execute();
   void setViscosity(Param nu);
   double getViscosity(unsigned int i = 0);
-  /// Coriolis term of a rotating frame (src/num/aslLBGK.cxx:207-208): not
-  /// available in the fused kernel yet; throws.
+  /// Coriolis term of a rotating frame (src/num/aslLBGK.cxx:207-208); an
+  /// empty Param switches it off.
   void setOmega(Param w);
   void setVectorTemplate(const VectorTemplate *vT);
   const VectorTemplate *getVectorTemplate() const { return vectorTemplate; }
@@ -81,8 +81,19 @@ class LBGKUtilities {
 };
 typedef std::shared_ptr<LBGKUtilities> SPLBGKUtilities;
 
-/// ASL's LBGKTurbulence reads fEq before assigning it (src/num/aslLBGK.cxx:
-/// 315-317): undefined behaviour, not provided.
+/// LBGKTurbulence (src/num/aslLBGK.h:163-171, src/num/aslLBGK.cxx:298-355):
+/// LBGK whose relaxation rate is limited per cell by the distance of the
+/// populations from equilibrium.  ASL evaluates that distance with fEq read
+/// before it is assigned (:315-317, an uninitialised private variable of the
+/// generated kernel); here fEq is the equilibrium of the cell's own density
+/// and momentum.
+class LBGKTurbulence : public LBGK {
+ public:
+  LBGKTurbulence(DataD v, Param nu, const VectorTemplate *vT);
+  LBGKTurbulence(Block b, Param nu, const VectorTemplate *vT, bool compVel = true, bool compRho = true,
+                 acl::CommandQueue queue = acl::hardware.defaultQueue);
+};
+typedef std::shared_ptr<LBGKTurbulence> SPLBGKTurbulence;
 
 }  // namespace asl
 

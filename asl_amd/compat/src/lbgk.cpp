@@ -69,6 +69,16 @@ LBGK::LBGK(DataD v, Param nu, const VectorTemplate *vT) : LBGK(v->getInternalBlo
   bind_view(*vel_, h_, type_, 1, 3, this);
 }
 
+LBGKTurbulence::LBGKTurbulence(Block b, Param nu, const VectorTemplate *vT, bool compVel, bool compRho,
+                               acl::CommandQueue queue)
+    : LBGK(b, nu, vT, compVel, compRho, queue) {
+  check_abi(aslhip_set_turbulence(h_, 1));
+}
+
+LBGKTurbulence::LBGKTurbulence(DataD v, Param nu, const VectorTemplate *vT) : LBGK(v, nu, vT) {
+  check_abi(aslhip_set_turbulence(h_, 1));
+}
+
 void LBGK::create(const Block &b, acl::TypeID t) {
   if (nD(b) != 3) errorMessage("LBGK: only 3D blocks are supported");
   const int q = q_of(vectorTemplate);

@@ -136,24 +136,59 @@ __device__ __forceinline__ void moments(const C (&v)[Q], C &r, C (&m)[3]) {
   for (int j = 1; j < Q; ++j) r = fmaT(v[j], C(LT::w(j)), r);
 }
 
+// LBGKTurbulence::init0 (src/num/aslLBGK.cxx:298-355): the relaxation of a
+// cell from how far its populations are from equilibrium,
+//   r = 4 |f - feq|^2 / |f|^2,   w = r > 1 ? 1 / (3 min(nu r, 1/2) + 1/2) : 1 / (3 nu + 1/2).
+// ASL reads feq there before assigning it (undefined in its generated code);
+// here feq is the equilibrium of the cell's own rho and momentum (before the
+// rotating-frame term), the value that statement is evidently meant to use.
+template <int Q, class C>
+__device__ __forceinline__ C turbulence_w(const C (&v)[Q], C r, const C (&m)[3], bool compressible, C nu, C w0) {
+  C mm = (C(1.5) * m[0]) * m[0];
+  mm = fmaT(C(1.5) * m[1], m[1], mm);
+  mm = fmaT(C(1.5) * m[2], m[2], mm);
+  const C mmr = compressible ? mm / r : mm;
+  C fl2 = v[0] * v[0];
+  C d = v[0] - (r - mmr);
+  C dl2 = d * d;
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    const C a = dot_c<Q>(m, i);
+    const C fe = compressible ? (fmaT(C(3.), a, r) + (C(4.5) * (a * a)) / r) - mmr
+                              : fmaT(C(4.5), a * a, fmaT(C(3.), a, r)) - mmr;
+    fl2 = fmaT(v[i], v[i], fl2);
+    const C di = v[i] - fe;
+    dl2 = fmaT(di, di, dl2);
+  }
+  const C rt = C(4.) * (dl2 / fl2);
+  const C nr = nu * rt;
+  const C lim = nr < C(.5) ? nr : C(.5);
+  return rt > C(1.) ? C(1.) / (C(3.) * lim + C(.5)) : w0;
+}
+
 // The BGK collision of one cell: ASL's LBGK::init0 (src/num/aslLBGK.cxx:
 // 187-235) with computeRho / computeMomentum / a_i_dot_v (:46-72).  Each
 // x*y + z of ASL's expression tree is one fma (OpenCL's default FP_CONTRACT
 // ON); the CPU oracle contracts the same sites.  Opposite directions share
 // (4.5 a^2)/rho: a_inv(i) = -a_i exactly in IEEE arithmetic, so this is the
 // same value ASL computes twice.  On return v holds the post-collision
-// populations, r the density and m the momentum.  CL: the closure network
-// replaces w per cell (FP32 collision only).
+// populations, r the density and m the momentum.  CL: a per-cell relaxation
+// replaces w: LBGKTurbulence's (turb), else the closure network's (D3Q19,
+// FP32 collision).
 template <int Q, class C, bool CL = false>
 __device__ __forceinline__ void collide(C (&v)[Q], C w, bool compressible, bool omega, const C (&om)[3], C &r_o,
                                         C (&m_o)[3], const ClosureParams *cl = nullptr, float tau0 = 0.f,
-                                        bool cl_mfma = false) {
+                                        bool cl_mfma = false, bool turb = false, C nu = C(0)) {
   C m[3], r;
   moments<Q, C>(v, r, m);
-  if constexpr (CL) {  // the learned closure sets the cell's relaxation
-    float x[7];
-    closure_features<Q>(v, r, m, x);
-    w = cl_mfma ? closure_relax(closure_y_mfma(x, *cl), *cl, tau0) : closure_w(x, *cl, tau0);
+  if constexpr (CL) {  // a per-cell relaxation: LBGKTurbulence or the learned closure
+    if (turb) {
+      w = turbulence_w<Q, C>(v, r, m, compressible, nu, w);
+    } else if constexpr (Q == 19 && sizeof(C) == 4) {
+      float x[7];
+      closure_features<Q>(v, r, m, x);
+      w = cl_mfma ? closure_relax(closure_y_mfma(x, *cl), *cl, tau0) : closure_w(x, *cl, tau0);
+    }
   }
   // LBGK::setOmega (src/num/aslLBGK.cxx:207-208): momentum += (2/w) omega x
   // momentum, one component after the other, each seeing the ones before it
@@ -378,7 +413,8 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L, void *
     for (int d = 0; d < 3; ++d) om[d] = sizeof(C) == 4 ? C(a.om32[d]) : C(a.om64[d]);
   }
   collide<Q, C, CL>(v, C(a.w), (a.flags & STEP_F_COMPRESSIBLE) != 0, omega, om, r, m, a.cl, a.cl_tau0,
-                    (a.flags & STEP_F_CLOSURE_MFMA) != 0);
+                    (a.flags & STEP_F_CLOSURE_MFMA) != 0, (a.flags & STEP_F_TURB) != 0,
+                    sizeof(C) == 4 ? C(a.nu32) : C(a.nu64));
 
   if (!live || !act) return;
   // Stores address every direction from fd[0] too (one scalar base pair
@@ -729,17 +765,12 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_final(const double *partial, i
 template <int Q, class S, bool ODD, bool PER>
 hipError_t step_w(bool wide, const StepLaunch &a, hipStream_t s) {
   const dim3 grid(a.nblocks), block(a.block);
-  if (a.closure) {
-    // the closure variants exist for D3Q19 with FP32 collision only
-    if constexpr (Q == 19 && sizeof(typename Codec<S>::C) == 4) {
-      if (wide)
-        hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true, true>), grid, block, 0, s, a.args);
-      else
-        hipLaunchKernelGGL((k_step<Q, S, ODD, PER, false, true>), grid, block, 0, s, a.args);
-      return hipGetLastError();
-    } else {
-      return hipErrorInvalidValue;
-    }
+  if (a.closure) {  // the per-cell relaxation variant (closure: D3Q19 FP32 only, checked by the host)
+    if (wide)
+      hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true, true>), grid, block, 0, s, a.args);
+    else
+      hipLaunchKernelGGL((k_step<Q, S, ODD, PER, false, true>), grid, block, 0, s, a.args);
+    return hipGetLastError();
   }
   if (wide)
     hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true, false>), grid, block, 0, s, a.args);

@@ -23,7 +23,8 @@ enum : unsigned {
   STEP_F_OMEGA = 16u,        // rotating frame: momentum += (2/w) omega x momentum
   STEP_F_CLOSURE = 32u,      // per-cell relaxation time from the closure network
   STEP_F_CLOSURE_MFMA = 64u, // ... evaluated with BF16 matrix instructions
-  STEP_F_COUNT = 128u,       // number of flag combinations
+  STEP_F_TURB = 128u,        // LBGKTurbulence's relaxation limiter
+  STEP_F_COUNT = 256u,       // number of flag combinations
 };
 
 // One fused collide+stream pass (AA pattern) over the ghost-inclusive block.
@@ -56,6 +57,8 @@ struct StepArgs {
   double om64[3];
   const ClosureParams *cl;    // closure network (STEP_F_CLOSURE)
   float cl_tau0;              // 3 nu + 1/2 in float
+  float nu32;                 // viscosity (STEP_F_TURB)
+  double nu64;
   const unsigned char *code;  // packed cell types
   const unsigned short *pat;  // link pattern of CODE_FLUID_BC / CODE_GHOST cells
   const LinkRec *rec;         // link patterns
@@ -157,7 +160,7 @@ struct StepLaunch {
   const StepArgs *args;
   int nblocks;
   int block;
-  bool closure;  // the closure variant of the kernel (D3Q19, FP32 collision)
+  bool closure;  // the per-cell relaxation variant (closure or LBGKTurbulence)
 };
 
 // wide: the populations span more than 4 GiB (64-bit per-lane addresses).

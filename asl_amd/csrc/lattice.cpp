@@ -133,7 +133,9 @@ struct aslhip_lattice {
   std::vector<StepArgs> h_args;  // host copy of d_args (empty: not uploaded yet)
   // hipGraph chunks of ASLHIP_GRAPH_CHUNK plain steps starting with an even /
   // odd step (the velocity-map link fills included); rebuilt after finalize
+  // and when the step flags change
   hipGraphExec_t graph[2] = {nullptr, nullptr};
+  unsigned graph_fl[2] = {0u, 0u};  // the step flags each graph was captured with
   int nvl = 0;
   // parameters
   double nu = 0.1;
@@ -142,6 +144,7 @@ struct aslhip_lattice {
   double omega[3] = {0., 0., 0.};
   // learned closure (network parameters in device memory)
   bool has_closure = false;
+  bool turbulence = false;   // LBGKTurbulence
   bool closure_mfma = true;  // evaluate the network with BF16 matrix instructions
   ClosureParams *d_cl = nullptr;
   float *d_feat = nullptr;  // closure features scratch (7 per interior cell)
@@ -201,7 +204,8 @@ void free_all(aslhip_lattice *h) {
 // StepArgs::flags bits fixed by the lattice's parameters.
 unsigned base_flags(const aslhip_lattice *h) {
   return (h->compressible ? STEP_F_COMPRESSIBLE : 0u) | (h->has_omega ? STEP_F_OMEGA : 0u) |
-         (h->has_closure ? STEP_F_CLOSURE : 0u) | (h->has_closure && h->closure_mfma ? STEP_F_CLOSURE_MFMA : 0u);
+         (h->has_closure ? STEP_F_CLOSURE : 0u) | (h->has_closure && h->closure_mfma ? STEP_F_CLOSURE_MFMA : 0u) |
+         (h->turbulence ? STEP_F_TURB : 0u);
 }
 
 double relaxation(const aslhip_lattice *h) {
@@ -601,6 +605,8 @@ StepArgs step_args(const aslhip_lattice *h, int block = 0) {
   }
   a.cl = h->d_cl;
   a.cl_tau0 = 3.f * static_cast<float>(h->nu) + .5f;
+  a.nu32 = static_cast<float>(h->nu);
+  a.nu64 = h->nu;
   a.flags = base_flags(h);
   return a;
 }
@@ -642,7 +648,7 @@ int launch_one(aslhip_lattice *h, unsigned fl, std::vector<hipEvent_t> *ev, int 
   a.nblocks = h->h_args[0].nblocks;
   a.block = h->h_args[0].block;
   a.args = h->d_args + fl;
-  a.closure = h->has_closure;
+  a.closure = h->has_closure || h->turbulence;
   if (ev) HIPCHK(hipEventRecord((*ev)[2 * s], h->stream));
   HIPCHK(launch_step(h->Q, h->dtype, h->next_odd, h->periodic, is_wide(h), a, h->stream));
   if (ev) HIPCHK(hipEventRecord((*ev)[2 * s + 1], h->stream));
@@ -654,10 +660,16 @@ int launch_one(aslhip_lattice *h, unsigned fl, std::vector<hipEvent_t> *ev, int 
 // Captures ASLHIP_GRAPH_CHUNK plain steps starting at parity odd.
 int build_graph(aslhip_lattice *h, bool odd) {
   hipGraphExec_t &g = h->graph[odd ? 1 : 0];
-  if (g) return 0;
+  const unsigned fl = base_flags(h);
+  if (g) {
+    if (h->graph_fl[odd ? 1 : 0] == fl) return 0;
+    // captured with other flags (frame, compressibility, relaxation model):
+    // its launches name the old kernel variant and argument slot
+    (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
   const bool keep = h->next_odd;
   h->next_odd = odd;
-  const unsigned fl = base_flags(h);
   hipGraph_t graph = nullptr;
   HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int r = 0;
@@ -671,7 +683,11 @@ int build_graph(aslhip_lattice *h, bool odd) {
   if (e != hipSuccess) return fail(ASLHIP_EHIP, "ASL ERROR: graph capture: %s", hipGetErrorString(e));
   e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
-  if (e != hipSuccess) return fail(ASLHIP_EHIP, "ASL ERROR: graph instantiate: %s", hipGetErrorString(e));
+  if (e != hipSuccess) {
+    g = nullptr;
+    return fail(ASLHIP_EHIP, "ASL ERROR: graph instantiate: %s", hipGetErrorString(e));
+  }
+  h->graph_fl[odd ? 1 : 0] = fl;
   return 0;
 }
 
@@ -865,6 +881,7 @@ int aslhip_closure_set(aslhip_lattice *h, const float *params, const float mu[7]
   if (int r = check(h)) return r;
   if (params && (h->Q != 19 || h->dtype == ASLHIP_F64))
     return fail(ASLHIP_EINVAL, "ASL ERROR: the closure needs a D3Q19 lattice with FP32 collision");
+  if (params && h->turbulence) return fail(ASLHIP_EINVAL, "ASL ERROR: LBGKTurbulence and the closure exclude each other");
   if (params && (!mu || !sc || !std::isfinite(out_scale) || out_scale < 0.f))
     return fail(ASLHIP_EINVAL, "ASL ERROR: closure_set: bad normalisation or output scale");
   if (int r = flush_deferred(h)) return r;  // a pending step runs with the old relaxation
@@ -886,6 +903,14 @@ int aslhip_closure_set(aslhip_lattice *h, const float *params, const float mu[7]
   HIPCHK(hipStreamSynchronize(h->stream));  // steps queued earlier read the old parameters
   HIPCHK(hipMemcpy(h->d_cl, &c, sizeof c, hipMemcpyHostToDevice));
   h->has_closure = true;
+  return 0;
+}
+
+int aslhip_set_turbulence(aslhip_lattice *h, int on) {
+  if (int r = check(h)) return r;
+  if (on && h->has_closure) return fail(ASLHIP_EINVAL, "ASL ERROR: LBGKTurbulence and the closure exclude each other");
+  if (int r = flush_deferred(h)) return r;
+  h->turbulence = on != 0;
   return 0;
 }
 
@@ -1404,7 +1429,10 @@ int batch_graph(aslhip_batch *b, bool odd) {
   int r = 0;
   for (int s = 0; s < ASLHIP_GRAPH_CHUNK && !r; ++s, o = !o) r = batch_launch(b, o, false);
   hipError_t e = hipStreamEndCapture(b->stream, &graph);
-  if (r) return r;
+  if (r) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r;
+  }
   if (e != hipSuccess) return fail(ASLHIP_EHIP, "ASL ERROR: graph capture: %s", hipGetErrorString(e));
   e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
@@ -1440,7 +1468,8 @@ int batch_step_impl(aslhip_batch *b, int nsteps, unsigned flags) {
     aslhip_lattice *h = b->lat[l];
     if (int r = prepare(h, false)) return r;
     if (h->nvl) return fail(ASLHIP_EINVAL, "ASL ERROR: velocity map BCs are not supported in batches");
-    if (h->has_closure) return fail(ASLHIP_EINVAL, "ASL ERROR: the closure is not supported in batches");
+    if (h->has_closure || h->turbulence)
+      return fail(ASLHIP_EINVAL, "ASL ERROR: per-cell relaxation (closure, turbulence) is not supported in batches");
     if (h->next_odd != odd) return fail(ASLHIP_ESTATE, "ASL ERROR: batch lattices are at different step parities");
     desc[l] = step_args(h, b->block);
     desc[n + l] = desc[l];

@@ -90,6 +90,10 @@ class Lattice:
         call("aslhip_closure_features", self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
         return out
 
+    def set_turbulence(self, on=True):
+        """LBGKTurbulence's relaxation limiter (with its equilibrium defined; see include/aslhip.h)."""
+        call("aslhip_set_turbulence", self.h, 1 if on else 0)
+
     def set_omega(self, omega):
         """LBGK::setOmega: angular velocity (3 components) of the rotating frame; None turns it off."""
         if omega is None:

@@ -120,6 +120,13 @@ int aslhip_set_viscosity(aslhip_lattice *h, double nu);
  * at src/num/aslLBGK.cxx:220-231. */
 int aslhip_set_compressible(aslhip_lattice *h, int compressible);
 
+/* asl::LBGKTurbulence — src/num/aslLBGK.h:163-171,
+ * src/num/aslLBGK.cxx:298-355: per cell, r = 4 |f - feq|^2 / |f|^2 and the
+ * relaxation 1 / (3 min(nu r, 1/2) + 1/2) where r > 1 (else the plain one).
+ * ASL reads feq there before assigning it (undefined); feq here is the
+ * equilibrium of the cell's rho and momentum.  on = 0 returns to asl::LBGK. */
+int aslhip_set_turbulence(aslhip_lattice *h, int on);
+
 /* asl::LBGK::setOmega(Param w) — src/num/aslLBGK.h:120,
  * src/num/aslLBGK.cxx:207-208,252-255.  Rotating frame with angular velocity
  * omega[3] (lattice units): each collision first sets

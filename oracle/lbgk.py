@@ -34,6 +34,7 @@ def load_library():
     lib.orc_set_viscosity.argtypes = [ctypes.c_void_p, ctypes.c_double]
     lib.orc_set_compressible.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_set_omega.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.orc_set_turbulence.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_set_closure.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]
     lib.orc_set_closure.restype = ctypes.c_int
     lib.orc_closure_features.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -156,6 +157,10 @@ class OracleLattice:
         if self.lib.orc_closure_features(self.h, out.ctypes.data):
             raise ValueError("closure features need a periodic D3Q19 lattice")
         return out
+
+    def set_turbulence(self, on=True):
+        """LBGKTurbulence (src/num/aslLBGK.cxx:298-355), fEq the equilibrium of the cell's moments."""
+        self.lib.orc_set_turbulence(self.h, 1 if on else 0)
 
     def set_omega(self, omega):
         """LBGK::setOmega: angular velocity (3 floats) of the rotating frame, None: off."""

@@ -162,6 +162,8 @@ struct Lattice {
   std::vector<float> cl;          // CL_NPARAM values, layout of closure_eval
   float cl_mu[7], cl_sc[7], cl_out = 0.f;
   int cl_mode = 1;  // 0: FP32 chain; 1: BF16 inputs / hidden activations, exact sums
+  // LBGKTurbulence (src/num/aslLBGK.cxx:298-355)
+  bool turb = false;
   double dx = 1., pos0[3] = {0., 0., 0.};
   std::vector<BC<T>> bcs;
   // per-direction constants as ASL prints them (double -> T)
@@ -308,6 +310,56 @@ struct Lattice {
     return T(1.) / T(tau < lo ? lo : tau);
   }
 
+  // aDotMomentum = momentum . c_i (a_i_dot_v, src/num/aslLBGK.cxx:65-72):
+  // the nonzero components of c_i in order, signed, summed left to right.
+  inline T c_dot(int i, const T *m) const {
+    bool first = true;
+    T a = 0;
+    for (int d = 0; d < 3; ++d) {
+      if (vt.c[i][d] == 0) continue;
+      T t = vt.c[i][d] > 0 ? m[d] : -m[d];
+      a = first ? t : a + t;
+      first = false;
+    }
+    return a;
+  }
+
+  // fEq (src/num/aslLBGK.cxx:333-345) of direction i; mmr = 1.5 m.m (/ rho).
+  inline T feq_dir(int i, T r, T a, T mmr) const {
+    if (i == 0) return r - mmr;  // rho + 3*0 + 4.5*0/rho - mm/rho
+    if (compressible) return (std::fma(T(3.), a, r) + (T(4.5) * (a * a)) / r) - mmr;
+    return std::fma(T(4.5), a * a, std::fma(T(3.), a, r)) - mmr;
+  }
+
+  inline T mm15(const T *m) const {
+    T mm = (T(1.5) * m[0]) * m[0];
+    mm = std::fma(T(1.5) * m[1], m[1], mm);
+    return std::fma(T(1.5) * m[2], m[2], mm);
+  }
+
+  // LBGKTurbulence::init0 (src/num/aslLBGK.cxx:312-318):
+  //   w = 4 (fOld - fEq).(fOld - fEq) / fOld.fOld,
+  //   w = w > 1 ? 1 / (3 min(nu w, .5) + .5) : 1 / (3 nu + .5).
+  // ASL evaluates it before fEq is assigned (an uninitialised private
+  // variable of the generated kernel); fEq here is the equilibrium of the
+  // cell's own rho and momentum, before the rotating-frame term.
+  inline T turbulence_w(const T *f, T r, const T *m, T w_plain) const {
+    const int Q = vt.Q;
+    const T mm = mm15(m);
+    const T mmr = compressible ? mm / r : mm;
+    T fl2 = f[0] * f[0];
+    T d0 = f[0] - feq_dir(0, r, T(0), mmr);
+    T dl2 = d0 * d0;
+    for (int i = 1; i < Q; ++i) {
+      fl2 = std::fma(f[i], f[i], fl2);
+      const T di = f[i] - feq_dir(i, r, c_dot(i, m), mmr);
+      dl2 = std::fma(di, di, dl2);
+    }
+    const T ratio = T(4.) * (dl2 / fl2);
+    const T lim = std::min(nu * ratio, T(.5));
+    return ratio > T(1.) ? T(1.) / (T(3.) * lim + T(.5)) : w_plain;
+  }
+
   inline void collide(T *f, T w, T &rho_o, T *v_o) const {
     const int Q = vt.Q;
     // momentum = computeMomentum(fOld) (:51-63), pRho = computeRho(fOld) (:46-49)
@@ -324,7 +376,9 @@ struct Lattice {
     }
     T r = f[0] * cw[0];
     for (int j = 1; j < Q; ++j) r = std::fma(f[j], cw[j], r);
-    if (has_closure) {
+    if (turb) {
+      w = turbulence_w(f, r, m, w);
+    } else if (has_closure) {
       T x[7];
       closure_features(f, r, m, x);
       w = closure_w(x, T(3.) * nu + T(.5));
@@ -346,29 +400,11 @@ struct Lattice {
     else
       for (int d = 0; d < 3; ++d) v_o[d] = m[d];
     // 1.5 * momentum * momentum (dot product), then / pRho (compressible)
-    T mm = (T(1.5) * m[0]) * m[0];
-    mm = std::fma(T(1.5) * m[1], m[1], mm);
-    mm = std::fma(T(1.5) * m[2], m[2], mm);
+    const T mm = mm15(m);
     const T one_w = T(1.) - w;
     const T mmr = compressible ? mm / r : mm;
     for (int i = 0; i < Q; ++i) {
-      // aDotMomentum = momentum . c_i (a_i_dot_v, :65-72)
-      bool first = true;
-      T a = 0;
-      for (int d = 0; d < 3; ++d) {
-        if (vt.c[i][d] == 0) continue;
-        T t = vt.c[i][d] > 0 ? m[d] : -m[d];
-        a = first ? t : a + t;
-        first = false;
-      }
-      T feq;
-      if (i == 0) {
-        feq = r - mmr;  // rho + 3*0 + 4.5*0/rho - mm/rho
-      } else if (compressible) {
-        feq = (std::fma(T(3.), a, r) + (T(4.5) * (a * a)) / r) - mmr;
-      } else {
-        feq = std::fma(T(4.5), a * a, std::fma(T(3.), a, r)) - mmr;
-      }
+      const T feq = feq_dir(i, r, i ? c_dot(i, m) : T(0), mmr);
       f[i] = std::fma(f[i], one_w, w * feq);  // fOld*(1-w) + w*fEq
     }
   }
@@ -755,6 +791,10 @@ int orc_set_closure(void *h, const float *params, const float *mu, const float *
 
 void orc_surface(void *h, const float *map, int flux, double *out) {
   dispatch(h, [&](auto &L) { L.surface(map, flux != 0, out); });
+}
+
+void orc_set_turbulence(void *h, int on) {
+  dispatch(h, [&](auto &L) { L.turb = on != 0; });
 }
 
 void orc_set_closure_mode(void *h, int mode) {

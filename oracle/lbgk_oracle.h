@@ -36,6 +36,8 @@ int orc_set_threads(int n);
 void orc_set_viscosity(void *h, double nu);
 void orc_set_compressible(void *h, int c);
 void orc_set_omega(void *h, const double *omega); /* NULL: off */
+/* LBGKTurbulence's relaxation (fEq: the equilibrium of the cell's moments). */
+void orc_set_turbulence(void *h, int on);
 /* Learned closure (FP32 D3Q19 lattices): 1345 parameters (BF16-rounded
  * here), input shift/scale (7 each), output scale; params NULL: off.
  * Returns -1 for other lattices. */

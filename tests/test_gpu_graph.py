@@ -102,3 +102,20 @@ def test_launch_tuning_and_autotune_keep_results():
     same(L, ref)
     with pytest.raises(asl_amd.ASLError):
         L.set_tuning(_lib.TUNE_BLOCK, 100)
+
+
+def test_graph_follows_flag_changes():
+    """A captured chunk names the kernel variant and argument slot of the
+    flags it was captured with; switching the rotating frame, the
+    compressibility or the turbulence limiter between replays must re-capture.
+    Checked against eager steps of a twin lattice."""
+    A, _ = cavity()
+    B, _ = cavity()
+    for setter, value in ((None, None), ("set_omega", (0.0, 0.0, 0.01)), ("set_compressible", False),
+                          ("set_turbulence", True), ("set_omega", None)):
+        if setter:
+            getattr(A, setter)(value)
+            getattr(B, setter)(value)
+        A.step(200, graph=True)
+        B.step(200)
+        same(A, B)

@@ -290,3 +290,59 @@ def test_surface_force_and_flux(dtype):
     assert np.abs(fo[(slice(None),) + inner]).max() > 1e-4
     assert np.array_equal(fg[(slice(None),) + inner], fo[(slice(None),) + inner].astype(fg.dtype))
     assert np.array_equal(qg[inner], qo[inner].astype(qg.dtype))
+
+
+@pytest.mark.parametrize("q", [15, 19, 27])
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+@pytest.mark.parametrize("compressible", [True, False])
+def test_turbulence_periodic(q, dtype, compressible):
+    """LBGKTurbulence on a periodic lattice started far from equilibrium (cells
+    in all three regimes of the limiter), GPU vs oracle bit for bit."""
+    size = (12, 10, 9)
+    S = tuple(s + 2 for s in size)
+    rng = np.random.default_rng(q)
+    f0 = 1.0 + rng.uniform(0.01, 1.6, S) * rng.standard_normal((q,) + S)
+    G, O = pair(size, q, nu=0.2, dtype=dtype, periodic=True, compressible=compressible)
+    G.set_turbulence(True)
+    O.set_turbulence(True)
+    G.write_f(f0)
+    O.write_f(G.f().astype(np.float64))
+    s = (slice(None),) + (slice(1, -1),) * 3
+    for n in (1, 24):
+        G.step(n)
+        O.run(n)
+        fo = O.f()[s].astype(G.f().dtype)
+        assert np.isfinite(fo).all()
+        assert np.array_equal(G.f()[s], fo)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_turbulence_sphere_channel_d3q15(dtype):
+    """The locomotive set-up in miniature: LBGKTurbulence, D3Q15, velocity map
+    tunnel + no-slip body + pressure-velocity inlet/outlet, bit for bit."""
+    size = (24, 14, 12)
+    body = geometry.normalize(geometry.sphere(size, 3.0, (9.0, 6.5, 5.5)))
+    G, O = pair(size, 15, nu=0.001, dtype=dtype)
+    G.set_turbulence(True)
+    O.set_turbulence(True)
+    G.init_equilibrium(1.0, (0.1, 0, 0))
+    O.init_f(1.0, (0.1, 0, 0))
+    bcs = [add_bc(G, O, _lib.BC_NOSLIP_MAP, mapv=body),
+           add_bc(G, O, _lib.BC_CONSTANT_PRESSURE_VELOCITY, _lib.X0 | _lib.XE, p=1.0, u=(0.1, 0, 0)),
+           add_bc(G, O, _lib.BC_NOSLIP, _lib.Y0 | _lib.YE | _lib.Z0 | _lib.ZE)]
+    run(G, O, 60, bcs)
+    d = compare(G, O)
+    assert d["rho"] == 0.0 and d["u"] == 0.0 and d["f"] == 0.0, d
+    G.set_turbulence(False)
+    O.set_turbulence(False)
+    G.step(10)
+    O.run(10, [o for _, o in bcs])
+    d = compare(G, O)
+    assert d["rho"] == 0.0 and d["u"] == 0.0 and d["f"] == 0.0, d
+
+
+def test_turbulence_and_closure_exclude_each_other():
+    G = asl_amd.Lattice((8, 8, 8), 19, periodic=True)
+    G.set_turbulence(True)
+    with pytest.raises(asl_amd.ASLError):
+        G.set_closure(np.zeros(_lib.CLOSURE_NPARAM, dtype=np.float32))

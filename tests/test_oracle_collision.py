@@ -229,3 +229,105 @@ def test_rotating_frame_off_is_plain_bgk():
         L.execute()
         out.append(L.f()[:, 1:-1, 1:-1, 1:-1])
     assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+def turbulence_rate(f, c, w_lat, nu, compressible):
+    """LBGKTurbulence's per-cell relaxation rate (numpy, FP64), with fEq the
+    textbook equilibrium of the cell's own moments (velocity form)."""
+    rho = np.einsum("i,i...->...", w_lat, f)
+    m = np.einsum("i,ia,i...->a...", w_lat, c, f)
+    d2 = np.zeros_like(rho)
+    for i in range(f.shape[0]):
+        if compressible:
+            u = m / rho
+            cu = np.einsum("a,a...->...", c[i], u)
+            feq = rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * (u * u).sum(0))
+        else:
+            cm = np.einsum("a,a...->...", c[i], m)
+            feq = rho + 3 * cm + 4.5 * cm * cm - 1.5 * (m * m).sum(0)
+        d2 += (f[i] - feq) ** 2
+    r = 4.0 * d2 / (f * f).sum(0)
+    plain = 1.0 / (3.0 * nu + 0.5)
+    return np.where(r > 1.0, 1.0 / (3.0 * np.minimum(nu * r, 0.5) + 0.5), plain), r
+
+
+@pytest.mark.parametrize("q", [15, 19, 27])
+@pytest.mark.parametrize("compressible", [True, False])
+def test_turbulence_one_step(q, compressible):
+    """LBGKTurbulence (src/num/aslLBGK.cxx:298-355): one step against a numpy
+    model that applies the textbook BGK with the per-cell rate
+    w = 1 / (3 min(nu r, 1/2) + 1/2) where r = 4 |f - feq|^2 / |f|^2 > 1.
+    The per-cell amplitudes put cells in all three regimes (plain rate,
+    limited rate, clamped at 1/2); a dropped factor 4, a swapped select or a
+    missing clamp moves many cells by far more than the tolerance."""
+    size, nu = (7, 6, 5), 0.2
+    c, w, _ = template(q)
+    rng = np.random.default_rng(40 + q)
+    L = OracleLattice(size, q, fp64=True, nu=nu, periodic=True)
+    L.set_compressible(compressible)
+    L.set_turbulence(True)
+    S = tuple(s + 2 for s in size)
+    amp = rng.uniform(0.01, 1.6, size)
+    f0 = np.full((q,) + S, np.nan)
+    f0[:, 1:-1, 1:-1, 1:-1] = 1.0 + amp * rng.standard_normal((q,) + size)
+    L.write_f(f0)
+    L.execute()
+    f = pull_periodic(f0[:, 1:-1, 1:-1, 1:-1], c)
+    wc, r = turbulence_rate(f, c, w, nu, compressible)
+    assert (r <= 1).sum() > 10 and ((r > 1) & (nu * r < 0.5)).sum() > 10 and (nu * r >= 0.5).sum() > 10
+    rho = np.einsum("i,i...->...", w, f)
+    m = np.einsum("i,ia,i...->a...", w, c, f)
+    ref = np.empty_like(f)
+    for i in range(q):
+        cm = np.einsum("a,a...->...", c[i], m)
+        if compressible:
+            feq = rho + 3 * cm + 4.5 * cm * cm / rho - 1.5 * (m * m).sum(0) / rho
+        else:
+            feq = rho + 3 * cm + 4.5 * cm * cm - 1.5 * (m * m).sum(0)
+        ref[i] = f[i] * (1 - wc) + wc * feq
+    got = L.f()[:, 1:-1, 1:-1, 1:-1]
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() < 1e-13 * scale
+
+
+@pytest.mark.parametrize("q", [15, 19, 27])
+def test_turbulence_near_equilibrium_is_plain_lbgk(q):
+    """Where every cell has r <= 1 the limiter keeps 1 / (3 nu + 1/2): the run
+    is bit-identical to plain LBGK (FP32)."""
+    size = (6, 5, 7)
+    S = tuple(s + 2 for s in size)
+    rng = np.random.default_rng(8)
+    u = 0.03 * rng.standard_normal((3,) + S)
+    runs = []
+    for turb in (False, True):
+        L = OracleLattice(size, q, fp64=False, nu=0.05, periodic=True)
+        L.set_turbulence(turb)
+        L.init_f(1.0, u)
+        L.run(30)
+        runs.append(L.f())
+    assert np.array_equal(runs[0], runs[1])
+
+
+def test_turbulence_limits_the_rate_far_from_equilibrium():
+    """A cell far from equilibrium relaxes with w = 1/(3 * 0.5 + 0.5) = 1/2 when
+    nu r >= 1/2, whatever nu is: two viscosities give the same step there."""
+    size = (4, 4, 4)
+    S = tuple(s + 2 for s in size)
+    q = 19
+    rng = np.random.default_rng(2)
+    f0 = np.full((q,) + S, np.nan)
+    f0[:, 1:-1, 1:-1, 1:-1] = 1.0 + rng.uniform(0.01, 3.0, size) * rng.standard_normal((q,) + size)
+    out = []
+    for nu in (0.3, 0.6):
+        L = OracleLattice(size, q, fp64=True, nu=nu, periodic=True)
+        L.set_turbulence(True)
+        L.write_f(f0)
+        L.execute()
+        out.append(L.f()[:, 1:-1, 1:-1, 1:-1])
+    c, w, _ = template(q)
+    f = pull_periodic(f0[:, 1:-1, 1:-1, 1:-1], c)
+    _, r = turbulence_rate(f, c, w, 0.3, True)
+    clamped = 0.3 * r >= 0.5
+    assert clamped.sum() > 5
+    np.testing.assert_array_equal(out[0][:, clamped], out[1][:, clamped])
+    assert not np.array_equal(out[0][:, ~clamped], out[1][:, ~clamped])
