@@ -55,6 +55,10 @@ SIGNATURES = {
     "aslhip_surface_force": [_vp, _fp, _vp],
     "aslhip_surface_flux": [_vp, _fp, _vp],
     "aslhip_template": [ctypes.c_int, _ip, _dp, _ip],
+    "aslhip_stl_count": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong)],
+    "aslhip_stl_read": [ctypes.c_char_p, _dp, ctypes.c_longlong],
+    "aslhip_surface_distance_map": [_dp, ctypes.c_longlong, _ip, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, _fp],
     "aslhip_read_cell_types": [_vp, _vp],
     "aslhip_reduce_conservation": [_vp, _dp],
     "aslhip_stats": [_vp, ctypes.POINTER(ctypes.c_int64)],

@@ -103,6 +103,22 @@ class BCMacroFillMap : public NumMethod {
   bool gradient;
 };
 
+/// Visualisation fill of a field with a position function at the map's BC
+/// nodes (map <= 0 and > -0.9999), on the host view: ASL's BCValuePFMap
+/// (src/num/aslBasicBC.cxx:114-148), behind generateBCVelocityVel
+/// (src/num/aslLBGKBC.cxx:847-852).
+class BCValuePFMap : public NumMethod {
+ public:
+  BCValuePFMap(SPAbstractDataWithGhostNodes data, SPPositionFunction v, SPAbstractDataWithGhostNodes map);
+  virtual void init() {}
+  virtual void execute();
+
+ private:
+  SPAbstractDataWithGhostNodes data;
+  SPPositionFunction value;
+  SPAbstractDataWithGhostNodes map;
+};
+
 // ComputeSurfaceFluxMap / ComputeSurfaceForceMap (src/num/aslLBGKBC.h:247-279):
 // execute() fills the field with the mass / momentum exchanged across the
 // links between the ghost and computation nodes of map (aslhip_surface_flux /
@@ -138,6 +154,7 @@ SPNumMethod generateBCConstantPressureVelocity(SPLBGK nm, double p, AVec<> v, SP
 SPNumMethod generateBCNoSlip(SPLBGK nm, SPAbstractDataWithGhostNodes map);
 SPNumMethod generateBCNoSlipVel(SPLBGK nmU, SPAbstractDataWithGhostNodes map);
 SPNumMethod generateBCNoSlipRho(SPLBGK nmU, SPAbstractDataWithGhostNodes map);
+SPNumMethod generateBCVelocityVel(SPLBGK nm, SPPositionFunction v, SPAbstractDataWithGhostNodes map);
 SPNumMethod generateBCVelocity(SPLBGK nm, SPPositionFunction v, SPAbstractDataWithGhostNodes map);
 SPNumMethod generateBCVelocity(SPLBGK nm, SPPositionFunction v, SPAbstractDataWithGhostNodes map,
                                SPAbstractDataWithGhostNodes computationalDomain);

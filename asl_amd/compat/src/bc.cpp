@@ -159,6 +159,44 @@ void BCMacroFillMap::execute() {
   }
 }
 
+BCValuePFMap::BCValuePFMap(SPAbstractDataWithGhostNodes d, SPPositionFunction v, SPAbstractDataWithGhostNodes m)
+    : data(d), value(v), map(m) {}
+
+// value(position) at the guarded map nodes; position = block position + dx * index
+// (Index2PositionACL, src/math/aslIndex2Position.cxx:64-108).  PFConstant or
+// PFRotationField axis x (position - centre) (src/math/aslPositionFunction.cxx:187-197).
+void BCValuePFMap::execute() {
+  if (!value) errorMessage("BCValuePFMap: no position function");
+  const Block &b = data->getBlock();
+  if (map->getBlock().getSize() != b.getSize()) errorMessage("BCValuePFMap: the map's block differs from the field's");
+  const unsigned int nc = data->getNComponents();
+  if (nc != 3) errorMessage("BCValuePFMap: a 3-component field is expected");
+  const std::vector<double> &m = map->component(0);
+  std::vector<std::vector<double>> v(nc);
+  for (unsigned int c = 0; c < nc; ++c) v[c] = data->component(c);
+  const int s0 = b.getSize()[0], s1 = b.getSize()[1], s2 = b.getSize()[2];
+  for (int i = 0; i < s0; ++i)
+    for (int j = 0; j < s1; ++j)
+      for (int k = 0; k < s2; ++k) {
+        const long g = (static_cast<long>(i) * s1 + j) * s2 + k;
+        const float mg = static_cast<float>(m[g]);
+        if (!(mg <= 0.f && mg > -.9999f)) continue;
+        const double p[3] = {b.position[0] + b.dx * i, b.position[1] + b.dx * j, b.position[2] + b.dx * k};
+        double u[3];
+        if (value->rotation) {
+          const double r[3] = {p[0] - value->centre[0], p[1] - value->centre[1], p[2] - value->centre[2]};
+          const AVec<double> &a = value->axis;
+          u[0] = a[1] * r[2] - a[2] * r[1];
+          u[1] = a[2] * r[0] - a[0] * r[2];
+          u[2] = a[0] * r[1] - a[1] * r[0];
+        } else {
+          for (int d = 0; d < 3; ++d) u[d] = value->constant[d];
+        }
+        for (unsigned int c = 0; c < nc; ++c) v[c][g] = u[c];
+      }
+  for (unsigned int c = 0; c < nc; ++c) data->values_[c] = v[c];
+}
+
 ComputeSurfaceFluxMap::ComputeSurfaceFluxMap(SPLBGK nm, SPDataWithGhostNodesACLData fF,
                                              SPAbstractDataWithGhostNodes m)
     : num(nm), field(fF), map(m) {}
@@ -246,6 +284,10 @@ SPNumMethod generateBCNoSlipVel(SPLBGK nmU, SPAbstractDataWithGhostNodes map) {
 
 SPNumMethod generateBCNoSlipRho(SPLBGK nmU, SPAbstractDataWithGhostNodes map) {
   return std::make_shared<BCMacroFillMap>(nmU, nmU->getRho(), map, true);
+}
+
+SPNumMethod generateBCVelocityVel(SPLBGK nm, SPPositionFunction v, SPAbstractDataWithGhostNodes map) {
+  return std::make_shared<BCValuePFMap>(nm->getVelocity(), v, map);
 }
 
 SPNumMethod generateBCVelocity(SPLBGK nm, SPPositionFunction v, SPAbstractDataWithGhostNodes map) {

@@ -3,6 +3,7 @@
 #include <iostream>
 
 #include <acl/aclTypes.h>
+#include <aslDataInc.h>
 #include <data/aslDataWithGhostNodes.h>
 #include <utilities/aslParametersManager.h>
 #include <writers/aslWriter.h>
@@ -19,6 +20,18 @@ ParametersManager *ParametersManager::current = nullptr;
 std::vector<float> AbstractData::componentAsFloat(unsigned int c) {
   const std::vector<double> &v = component(c);
   return std::vector<float>(v.begin(), v.end());
+}
+
+void initData(SPAbstractData d, double a) {
+  if (!d) errorMessage("initData: the container is not initialized");
+  for (auto &c : d->values_) std::fill(c.begin(), c.end(), a);
+}
+
+void initData(SPAbstractData d, const AVec<double> &a) {
+  if (!d) errorMessage("initData: the container is not initialized");
+  if (a.getSize() != d->getNComponents())
+    errorMessage("initData: the vector size differs from the number of components");
+  for (unsigned int c = 0; c < a.getSize(); ++c) std::fill(d->values_[c].begin(), d->values_[c].end(), a[c]);
 }
 
 SPDataWithGhostNodesACLData generateDataContainerACL_SP(const Block &b, acl::TypeID t, unsigned int n,

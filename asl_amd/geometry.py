@@ -81,3 +81,66 @@ def intersection(a, b):
 
 def complement(a):
     return -a
+
+
+# --- triangulated surfaces (asl::readSurface) ------------------------------
+
+def read_stl(path):
+    """Triangles of a binary or ASCII STL file, shape (n, 3, 3) float64 in the
+    file's x y z (values rounded to float, as VTK's reader keeps them):
+    the reading half of ``readSurface`` (``src/readers/aslVTKFormatReaders.cxx:266-305``)."""
+    import ctypes
+
+    from . import _lib
+    p = str(path).encode()
+    n = ctypes.c_longlong(0)
+    _lib.call("aslhip_stl_count", p, ctypes.byref(n))
+    tri = np.empty((n.value, 3, 3), dtype=np.float64)
+    if n.value:
+        _lib.call("aslhip_stl_read", p, tri.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n.value)
+    return tri
+
+
+def surface_map(tri, shape, pos, h, cap, shift, div1, div2):
+    """Distance map of triangles (n, 3, 3) in ASL axis order on the nodes
+    ``pos + h * index`` of a grid of ``shape`` (on the GPU; see
+    ``aslhip_surface_distance_map``): float32 array of that shape."""
+    import ctypes
+
+    from . import _lib
+    t = np.ascontiguousarray(np.asarray(tri, dtype=np.float64).reshape(-1))
+    sz = np.ascontiguousarray(np.asarray(shape, dtype=np.int32).reshape(3))
+    pa = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(3))
+    ha = np.ascontiguousarray(np.broadcast_to(np.asarray(h, dtype=np.float64), (3,)))
+    out = np.empty(tuple(int(s) for s in sz), dtype=np.float32)
+    dp = ctypes.POINTER(ctypes.c_double)
+    _lib.call("aslhip_surface_distance_map", t.ctypes.data_as(dp), t.size // 9,
+              sz.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), pa.ctypes.data_as(dp), ha.ctypes.data_as(dp),
+              float(cap), float(shift), float(div1), float(div2), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return out
+
+
+def stl_to_asl_axes(tri):
+    """readSurface's grid takes the file's x as ASL's last axis: (x, y, z) -> (z, y, x)."""
+    return np.asarray(tri)[..., ::-1]
+
+
+def block_nodes(size, dx, position):
+    """Node origin and spacing per axis of readSurface(file, Block) on the
+    ghost-inclusive grid of ``size`` nodes at ``position``: VTK's model bounds
+    [position, position + (size - 1) dx] over size - 1 intervals
+    (``src/readers/aslVTKFormatReaders.cxx:155-183``)."""
+    pos = np.asarray(position, dtype=np.float64)
+    s = np.asarray(size, dtype=np.float64)
+    top = pos + (s - 1.0) * dx
+    return pos, (top - pos) / (s - 1.0)
+
+
+def read_surface(path, size, dx, position=(0.0, 0.0, 0.0)):
+    """``readSurface(file, Block b)``: the map (float32, shape ``size``, the
+    ghost-inclusive grid of the lattice whose internal block is ``size - 2``
+    at ``position + dx``) of the STL surface:
+    ``(d - 1.2 dx) / dx / 1.8`` with d the distance capped at 3.6 dx."""
+    tri = stl_to_asl_axes(read_stl(path))
+    pos, h = block_nodes(size, dx, position)
+    return surface_map(tri, size, pos, h, 3.6 * dx, 1.2 * dx, dx, 1.8)

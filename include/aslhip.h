@@ -195,6 +195,26 @@ int aslhip_synchronize(aslhip_lattice *h);
 int aslhip_surface_force(aslhip_lattice *h, const float *map, void *out);
 int aslhip_surface_flux(aslhip_lattice *h, const float *map, void *out);
 
+/* asl::readSurface(file, ...) — src/readers/aslVTKFormatReaders.h:66-78,
+ * src/readers/aslVTKFormatReaders.cxx:266-305: the triangles of a binary or
+ * ASCII STL file, 9 doubles each (3 vertices, the file's x y z), rounded to
+ * float as VTK's reader stores them.  count gives the number, read fills
+ * ntri * 9 doubles. */
+int aslhip_stl_count(const char *path, long long *ntri);
+int aslhip_stl_read(const char *path, double *tri, long long ntri);
+
+/* surfaceToData — src/readers/aslVTKFormatReaders.cxx:132-217 (VTK's
+ * vtkImplicitModeller, capped, then rescaled): on a grid of size[3] nodes,
+ * node (i, j, k) at pos + h * (i, j, k) (ASL axis order, last index fastest),
+ *   d2  = min(cap^2, min over triangles of |node - T|^2 rounded to float),
+ *   map = ((sqrt(d2) - shift) / div1) / div2  in float (constants rounded to float).
+ * ASL uses cap = 3.6 dx, div1 = dx, div2 = 1.8 and shift = 1.2 dx (Block
+ * variant) or 1.8 dx (dx variants).  tri: ntri * 9 doubles in ASL axis order
+ * (readSurface's grid takes the file's x as ASL's last axis); map: host,
+ * size[0] * size[1] * size[2] floats. */
+int aslhip_surface_distance_map(const double *tri, long long ntri, const int size[3], const double pos[3],
+                                const double h[3], double cap, double shift, double div1, double div2, float *map);
+
 /* Writes the populations from a host view in the layout aslhip_read_f
  * returns (Q * len elements of the host element type, f_i of each cell,
  * weight-normalised as ASL stores them) and sets rho and u from them: the

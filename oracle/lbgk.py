@@ -35,6 +35,11 @@ def load_library():
     lib.orc_set_compressible.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_set_omega.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     lib.orc_set_turbulence.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_triangle_d2.restype = ctypes.c_double
+    lib.orc_triangle_d2.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.orc_surface_distance_map.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_void_p]
     lib.orc_set_closure.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]
     lib.orc_set_closure.restype = ctypes.c_int
     lib.orc_closure_features.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -83,6 +88,26 @@ def template(q):
     if n < 0:
         raise ValueError("unknown template %r" % (q,))
     return (np.array(c[: 3 * n], dtype=np.int64).reshape(n, 3), np.array(w[:n]), np.array(inv[:n], dtype=np.int64))
+
+
+def triangle_d2(p, tri):
+    """Squared distance from point p to the triangle tri (3 x 3 vertices)."""
+    pa = np.ascontiguousarray(np.asarray(p, dtype=np.float64).reshape(3))
+    ta = np.ascontiguousarray(np.asarray(tri, dtype=np.float64).reshape(9))
+    return float(load_library().orc_triangle_d2(pa.ctypes.data, ta.ctypes.data))
+
+
+def surface_distance_map(tri, size, pos, h, cap, shift, div1, div2):
+    """readSurface's map of triangles tri ((n, 3, 3), ASL axis order) on a grid of
+    size nodes at pos + h * index, float32 of shape size."""
+    t = np.ascontiguousarray(np.asarray(tri, dtype=np.float64).reshape(-1))
+    sz = np.ascontiguousarray(np.asarray(size, dtype=np.int32).reshape(3))
+    pa = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(3))
+    ha = np.ascontiguousarray(np.broadcast_to(np.asarray(h, dtype=np.float64), (3,)))
+    out = np.empty(tuple(int(s) for s in sz), dtype=np.float32)
+    load_library().orc_surface_distance_map(t.ctypes.data, t.size // 9, sz.ctypes.data, pa.ctypes.data, ha.ctypes.data,
+                                            float(cap), float(shift), float(div1), float(div2), out.ctypes.data)
+    return out
 
 
 def set_threads(n):

@@ -67,6 +67,13 @@ void orc_map_masks(void *h, int id, uint8_t *out);
 void orc_conservation(void *h, const uint8_t *mask, double out[5]);
 int orc_template(int q, int *c, double *w, int *inv);
 
+/* Surface distance maps (surface_map.cpp): squared point-triangle distance,
+ * and readSurface's map of ntri triangles (9 doubles each) on a grid of
+ * size[3] nodes at pos + h * index: map = ((sqrt(min(cap^2, d^2)) - shift) / div1) / div2. */
+double orc_triangle_d2(const double p[3], const double t[9]);
+void orc_surface_distance_map(const double *tri, long long ntri, const int size[3], const double pos[3],
+                              const double h[3], double cap, double shift, double div1, double div2, float *map);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,8 +48,23 @@ def test_cpp_sphere_channel_full_size():
     assert "bit-identical: yes" in p.stdout and "cell types match masks: yes" in p.stdout
 
 
+@pytest.mark.gpu
+def test_cpp_stl_body_turbulence(tmp_path):
+    # locomotive-style: STL body via readSurface(file, Block), LBGKTurbulence D3Q15,
+    # tunnel velocity map, visualisation BCs, surface force, .vti output
+    from test_gpu_surface_map import icosphere
+    from test_oracle_surface_map import write_stl_binary
+    path = tmp_path / "body.stl"
+    write_stl_binary(path, icosphere(0.8, (2.6, 1.5, 3.0)))  # file (x, y, z) = ASL (z, y, x)
+    p = _run("stl_body_parity", path, 200, tmp_path / "out")
+    assert p.returncode == 0
+    assert "map bit-identical: yes" in p.stdout and ", bit-identical: yes" in p.stdout
+    assert (tmp_path / "out_0.vti").stat().st_size > 10000
+
+
 @pytest.mark.skipif(not os.path.isdir(REF_FLOW), reason="ASL source tree not mounted")
-@pytest.mark.parametrize("driver", ["flow", "flow2", "flow3", "flowRotatingCylinders"])
+@pytest.mark.parametrize("driver", ["flow", "flow2", "flow3", "flowRotatingCylinders", "locomotive",
+                                    "locomotive_laminar", "locomotive_stability", "bus_wind", "compressor"])
 def test_asl_flow_drivers_compile_unchanged(driver, tmp_path):
     cmd = ["g++", "-std=c++17", "-O0", "-Wno-ignored-qualifiers", "-I", os.path.join(ROOT, "asl_amd", "compat", "include"),
            "-I", os.path.join(ROOT, "include"), os.path.join(REF_FLOW, driver + ".cc"), "-o", str(tmp_path / driver),
