@@ -197,6 +197,21 @@ class Lattice:
         call("aslhip_surface_flux", self.h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), out.ctypes.data)
         return out.reshape(self.S)
 
+    def save_abd(self, path, dx=1.0):
+        """Population checkpoint in ABD encoding (``asl_amd.abd``): name "f", the
+        ghost-inclusive block, q components (the host view of ``f()``)."""
+        from .abd import write_abd
+        write_abd(path, "f", self.f(), dx=dx)
+
+    def load_abd(self, path):
+        """Restore populations saved by ``save_abd`` (through ``write_f``)."""
+        from .abd import read_abd
+        name, _, size, _, comp = read_abd(path)
+        if name != "f" or tuple(size) != tuple(self.S) or comp.shape[0] != self.q:
+            raise ValueError("%s holds %r over %s x %d, not this lattice's populations" % (path, name, size,
+                                                                                         comp.shape[0]))
+        self.write_f(comp.astype(self.host_dtype))
+
     def write_f(self, f):
         """Set the populations (shape (q,) + S, ASL's weight-normalised f_i); rho and u follow."""
         a = np.ascontiguousarray(np.asarray(f, dtype=self.host_dtype).reshape(self.q * self.len))
