@@ -9,10 +9,6 @@ namespace aslhip {
 
 // Threads per workgroup of the step kernel (each thread one cell).
 constexpr int STEP_BLOCK = 1024;
-// Chunks per workgroup of the batched step kernel.  Runs of 4 were measured
-// slower on the 64-cavity batch (22.6 against 29.9 GLUPS): fewer workgroups,
-// and a wavefront's chunks do not overlap each other.
-constexpr int BATCH_RUN = 1;
 
 // Step-kernel flags.
 enum : unsigned {
@@ -74,7 +70,7 @@ struct StepArgs {
 // chunk b - first[lat[b]].
 struct BatchLaunch {
   const StepArgs *desc;
-  const uint2 *ent;  // per workgroup: lattice | chunks << 16, first chunk
+  const uint2 *ent;  // per workgroup: lattice, chunk
   int nblocks;
   int block;
 };

@@ -1367,9 +1367,15 @@ struct aslhip_batch {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   StepArgs *desc = nullptr;  // [2][n]: plain steps, then steps writing rho/u
-  uint2 *ent = nullptr;      // per workgroup: lattice | chunks << 16, first chunk
+  uint2 *ent = nullptr;      // per workgroup: lattice, chunk
   int nblocks = 0;
   int block = STEP_BLOCK;    // workgroup size of all lattices of the batch
+  // Graph chunks go group by group: each group of consecutive lattices (at
+  // most group_mb MiB of populations; 0: one group) takes all the chunk's
+  // steps before the next starts, so a group that fits the 256 MiB Infinity
+  // Cache is stepped from cache.  Entry ranges [first, second) of each group.
+  int group_mb = 0;
+  std::vector<std::pair<int, int>> groups;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // chunks starting with an even / odd step
   std::vector<StepArgs> hdesc;                    // host copy of desc (source of the async upload)
 };
@@ -1386,11 +1392,12 @@ void batch_free(aslhip_batch *b) {
   if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
-int batch_launch(aslhip_batch *b, bool odd, bool macro) {
+// One step of the lattices whose workgroup entries are [e0, e1).
+int batch_launch(aslhip_batch *b, bool odd, bool macro, int e0, int e1) {
   BatchLaunch L;
   L.desc = b->desc + (macro ? b->lat.size() : 0);
-  L.ent = b->ent;
-  L.nblocks = b->nblocks;
+  L.ent = b->ent + e0;
+  L.nblocks = e1 - e0;
   L.block = b->block;
   HIPCHK(launch_step_batch(b->Q, b->dtype, odd, b->periodic, b->wide, L, b->stream));
   return 0;
@@ -1399,13 +1406,23 @@ int batch_launch(aslhip_batch *b, bool odd, bool macro) {
 // The workgroup table for the batch's workgroup size (the lattices in order).
 int batch_tables(aslhip_batch *b) {
   std::vector<uint2> ent;
+  b->groups.clear();
+  const double budget = 1048576.0 * b->group_mb;
+  size_t gfirst = 0;
+  double gbytes = 0.;
   for (size_t l = 0; l < b->lat.size(); ++l) {
-    const StepArgs a = step_args(b->lat[l], b->block);
-    for (int c = 0; c < a.nblocks; c += BATCH_RUN) {
-      const int n = std::min(BATCH_RUN, a.nblocks - c);
-      ent.push_back(make_uint2(static_cast<unsigned>(l) | (static_cast<unsigned>(n) << 16), static_cast<unsigned>(c)));
+    const aslhip_lattice *h = b->lat[l];
+    const double bytes = static_cast<double>(h->Q) * h->selem * h->len;
+    if (b->group_mb > 0 && ent.size() > gfirst && gbytes + bytes > budget) {
+      b->groups.emplace_back(static_cast<int>(gfirst), static_cast<int>(ent.size()));
+      gfirst = ent.size();
+      gbytes = 0.;
     }
+    gbytes += bytes;
+    const StepArgs a = step_args(b->lat[l], b->block);
+    for (int c = 0; c < a.nblocks; ++c) ent.push_back(make_uint2(static_cast<unsigned>(l), static_cast<unsigned>(c)));
   }
+  b->groups.emplace_back(static_cast<int>(gfirst), static_cast<int>(ent.size()));
   for (auto &g : b->graph)
     if (g) {
       (void)hipGraphExecDestroy(g);
@@ -1425,9 +1442,11 @@ int batch_graph(aslhip_batch *b, bool odd) {
   if (g) return 0;
   hipGraph_t graph = nullptr;
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  bool o = odd;
   int r = 0;
-  for (int s = 0; s < ASLHIP_GRAPH_CHUNK && !r; ++s, o = !o) r = batch_launch(b, o, false);
+  for (const auto &g : b->groups) {
+    bool o = odd;
+    for (int s = 0; s < ASLHIP_GRAPH_CHUNK && !r; ++s, o = !o) r = batch_launch(b, o, false, g.first, g.second);
+  }
   hipError_t e = hipStreamEndCapture(b->stream, &graph);
   if (r) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -1491,7 +1510,7 @@ int batch_step_impl(aslhip_batch *b, int nsteps, unsigned flags) {
     }
   }
   for (; left > 0; --left, o = !o)
-    if (int r = batch_launch(b, o, macro && left == 1)) return r;
+    if (int r = batch_launch(b, o, macro && left == 1, 0, b->nblocks)) return r;
   for (auto *h : b->lat) h->next_odd = (nsteps % 2) ? !odd : odd;
   // the lattices' own streams see the result in order
   HIPCHK(hipEventRecord(b->done, b->stream));
@@ -1539,7 +1558,14 @@ int aslhip_batch_create(aslhip_batch **out, aslhip_lattice *const *lattices, int
 
 int aslhip_batch_set_tuning(aslhip_batch *b, int knob, int value) {
   if (!b) return fail(ASLHIP_EINVAL, "ASL ERROR: null batch handle");
-  if (knob != ASLHIP_TUNE_BLOCK) return fail(ASLHIP_EINVAL, "ASL ERROR: batches tune the workgroup size only");
+  if (knob == ASLHIP_TUNE_GROUP_MB) {
+    if (value < 0 || value > (1 << 20)) return fail(ASLHIP_EINVAL, "ASL ERROR: group size %d MiB out of 0..2^20", value);
+    if (value == b->group_mb) return 0;
+    b->group_mb = value;
+    return batch_tables(b);
+  }
+  if (knob != ASLHIP_TUNE_BLOCK)
+    return fail(ASLHIP_EINVAL, "ASL ERROR: batches tune the workgroup size and the group size only");
   if (value < 64 || value > STEP_BLOCK || value % 64)
     return fail(ASLHIP_EINVAL, "ASL ERROR: workgroup size %d is not a multiple of 64 in 64..%d", value, STEP_BLOCK);
   if (value == b->block) return 0;
@@ -1564,6 +1590,27 @@ int aslhip_batch_autotune(aslhip_batch *b, int nsteps, int *best_block) {
   }
   if (int r = aslhip_batch_set_tuning(b, ASLHIP_TUNE_BLOCK, bb)) return r;
   if (best_block) *best_block = bb;
+  return 0;
+}
+
+int aslhip_batch_autotune_groups(aslhip_batch *b, int chunks, int *best_mb) {
+  if (!b) return fail(ASLHIP_EINVAL, "ASL ERROR: null batch handle");
+  if (chunks < 1) return fail(ASLHIP_EINVAL, "ASL ERROR: autotune needs at least one chunk per candidate");
+  double best = 1e300;
+  int bm = b->group_mb;
+  for (int mb : {0, 64, 128, 160, 192, 224, 256, 384}) {
+    if (int r = aslhip_batch_set_tuning(b, ASLHIP_TUNE_GROUP_MB, mb)) return r;
+    // the first chunk captures the graphs; the timed ones replay them
+    if (int r = batch_step_impl(b, ASLHIP_GRAPH_CHUNK, ASLHIP_STEP_GRAPH)) return r;
+    double ms = 0.;
+    if (int r = aslhip_batch_time_steps(b, chunks * ASLHIP_GRAPH_CHUNK, ASLHIP_STEP_GRAPH, &ms)) return r;
+    if (ms < best) {
+      best = ms;
+      bm = mb;
+    }
+  }
+  if (int r = aslhip_batch_set_tuning(b, ASLHIP_TUNE_GROUP_MB, bm)) return r;
+  if (best_mb) *best_mb = bm;
   return 0;
 }
 

@@ -319,6 +319,13 @@ class Batch:
         call("aslhip_batch_autotune", self.h, int(nsteps), ctypes.byref(b))
         return b.value
 
+    def autotune_groups(self, chunks=2):
+        """Time the lattice-group sizes (MiB of populations stepped a whole graph
+        chunk at a time) over `chunks` real 100-step chunks each; keep the fastest."""
+        m = ctypes.c_int(0)
+        call("aslhip_batch_autotune_groups", self.h, int(chunks), ctypes.byref(m))
+        return m.value
+
 
 # ---------------------------------------------------------------------------
 # ASL-named layer (src/num/aslLBGK.h, src/num/aslLBGKBC.h)

@@ -265,6 +265,7 @@ def batch_study(steps, warmup):
     B = asl_amd.Batch(lat)
     B.step(max(warmup, 1), write_macro=False)
     block = B.autotune(10)
+    group_mb = B.autotune_groups(2)
     ms_batch = B.time_steps(steps, graph=True)
     solo = [batch_cavity(n, t) for n, t in specs]
     for L in solo:
@@ -289,6 +290,7 @@ def batch_study(steps, warmup):
     print(json.dumps({"study": "batched lattices", "lattices": len(specs), "cells": cells,
                       "sizes": [n for n, _ in specs], "steps": steps,
                       "batch_graph_mlups": round(cells * steps / ms_batch / 1e3, 1), "batch_block": block,
+                      "batch_group_mb": group_mb,
                       "batch_gbps": round(152.0 * cells * steps / ms_batch / 1e6, 1),
                       "per_lattice_streams_mlups": round(cells * steps / ms_solo / 1e3, 1),
                       "per_lattice_interleaved_mlups": round(cells * steps / ms_interleaved / 1e3, 1)}), flush=True)

@@ -251,6 +251,10 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
  *   1024); a workgroup takes that many consecutive cells of a plane. */
 #define ASLHIP_TUNE_XCD_REMAP 1
 #define ASLHIP_TUNE_BLOCK 2
+/* Batches only: graph chunks step groups of consecutive lattices of at most
+ * this many MiB of populations one group at a time (0: all lattices each
+ * step), so that a group can stay in the 256 MiB Infinity Cache. */
+#define ASLHIP_TUNE_GROUP_MB 3
 int aslhip_set_tuning(aslhip_lattice *h, int knob, int value);
 
 /* The autotuner: times nsteps (rounded up to even) real steps of the lattice
@@ -325,10 +329,13 @@ int aslhip_batch_destroy(aslhip_batch *b);
 int aslhip_batch_step(aslhip_batch *b, int nsteps, unsigned flags);
 /* As aslhip_batch_step, timed with HIP events on the batch's stream. */
 int aslhip_batch_time_steps(aslhip_batch *b, int nsteps, unsigned flags, double *ms_total);
-/* Workgroup size of the batch's launches (ASLHIP_TUNE_BLOCK only), and the
- * autotuner choosing it (256/512/1024) by timing nsteps real steps each. */
+/* Workgroup size (ASLHIP_TUNE_BLOCK) or group size (ASLHIP_TUNE_GROUP_MB)
+ * of the batch's launches; the autotuners choose the workgroup size
+ * (256/512/1024, timing nsteps real eager steps each) and the group size
+ * (0, 64 .. 384 MiB, timing `chunks` real graph chunks each). */
 int aslhip_batch_set_tuning(aslhip_batch *b, int knob, int value);
 int aslhip_batch_autotune(aslhip_batch *b, int nsteps, int *best_block);
+int aslhip_batch_autotune_groups(aslhip_batch *b, int chunks, int *best_mb);
 
 #ifdef __cplusplus
 }

@@ -92,3 +92,22 @@ def test_batch_workgroup_size_keeps_results():
     B.step(130 - 60 - 18, graph=True)
     for L, R in zip(members, solo):
         assert np.array_equal(interior(L.f()), interior(R.f()))
+
+
+@pytest.mark.parametrize("group_mb", [1, 3])
+def test_lattice_groups_keep_results(group_mb):
+    """Graph chunks that step groups of lattices one group at a time (here
+    groups of one to a few of these small lattices) end bit for bit where the
+    single lattices end."""
+    steps = 300
+    solo = [cavity(n, t) for n, t in SPECS]
+    for L in solo:
+        L.step(steps)
+    members = [cavity(n, t) for n, t in SPECS]
+    B = asl_amd.Batch(members)
+    B.set_tuning(_lib.TUNE_GROUP_MB, group_mb)
+    B.step(steps, graph=True)
+    for L, R in zip(members, solo):
+        assert np.array_equal(interior(L.f()), interior(R.f()))
+        assert np.array_equal(interior(L.rho()), interior(R.rho()))
+    assert B.autotune_groups(1) in (0, 64, 128, 160, 192, 224, 256, 384)
