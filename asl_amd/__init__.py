@@ -7,6 +7,8 @@ Importing works without a GPU; creating a lattice needs the built library
 and a GPU and raises otherwise.
 """
 from ._lib import ASLError, device_info, load_library  # noqa: F401
-from .lattice import (LBGK, Batch, Block, Lattice, LBGKUtilities, Mlp, NumMethod, VectorTemplate, d3q15, d3q19,  # noqa: F401
-                      d3q27, executeAll, generateBCConstantPressure, generateBCConstantPressureVelocity,
-                      generateBCConstantVelocity, generateBCNoSlip, generateBCVelocity, initAll)
+from .lattice import (LBGK, Batch, Block, Lattice, LBGKTurbulence, LBGKUtilities, Mlp, NumMethod,  # noqa: F401
+                      VectorTemplate, d3q15, d3q19, d3q27, executeAll, generateBCConstantPressure,
+                      generateBCConstantPressureVelocity, generateBCConstantVelocity, generateBCNoSlip,
+                      generateBCVelocity, generateComputeSurfaceFlux, generateComputeSurfaceForce, initAll,
+                      readSurface)

@@ -402,6 +402,10 @@ class LBGK(NumMethod):
     def setCompressible(self, flag=True):
         self.lattice.set_compressible(flag)
 
+    def setOmega(self, w):
+        """Rotating frame (src/num/aslLBGK.cxx:207-208); None switches it off."""
+        self.lattice.set_omega(w)
+
     def execute(self):
         self.lattice.step(1, defer=True)
 
@@ -416,6 +420,15 @@ class LBGK(NumMethod):
 
     def getVectorTemplate(self):
         return self.vectorTemplate
+
+
+class LBGKTurbulence(LBGK):
+    """``asl::LBGKTurbulence`` (src/num/aslLBGK.h:163-171): LBGK with the per-cell
+    limiter of the relaxation rate, fEq being the cell's own equilibrium."""
+
+    def __init__(self, block, nu, vt=None, compVel=True, compRho=True, dtype="f32", periodic=False):
+        super().__init__(block, nu, vt, compVel, compRho, dtype, periodic)
+        self.lattice.set_turbulence(True)
 
 
 class LBGKUtilities:
@@ -478,3 +491,36 @@ def generateBCConstantPressureVelocity(lbgk, p, v, sl_or_map):
 
 def generateBCVelocity(lbgk, v, mapv, computationalDomain=None, rotation=None):
     return _BC(lbgk, lbgk.lattice.add_map_bc(BC_VELOCITY_MAP, mapv, cdomain=computationalDomain, u=v, rot=rotation))
+
+
+class _SurfaceIntegral(NumMethod):
+    """ComputeSurfaceForceMap / ComputeSurfaceFluxMap (src/num/aslLBGKBC.h:247-279):
+    execute() fills ``field`` ((3,) + S force, or S flux) from the current state."""
+
+    def __init__(self, lbgk, mapv, flux):
+        self.num = lbgk
+        self.map = np.asarray(mapv, dtype=np.float32)
+        self.flux = flux
+        self.field = None
+
+    def init(self):
+        pass
+
+    def execute(self):
+        L = self.num.lattice
+        self.field = L.surface_flux(self.map) if self.flux else L.surface_force(self.map)
+
+
+def generateComputeSurfaceForce(lbgk, mapv):
+    return _SurfaceIntegral(lbgk, mapv, False)
+
+
+def generateComputeSurfaceFlux(lbgk, mapv):
+    return _SurfaceIntegral(lbgk, mapv, True)
+
+
+def readSurface(path, block):
+    """``asl::readSurface(file, Block b)`` for an STL file: the map on the nodes of
+    b (its ghost-inclusive grid), ``(d - 1.2 dx) / dx / 1.8`` with d capped at 3.6 dx."""
+    from . import geometry
+    return geometry.read_surface(path, block.size, block.dx, block.position)

@@ -111,3 +111,61 @@ def test_stl_body_in_a_channel(tmp_path, dtype):
     run(G, O, 40, bcs)
     d = compare(G, O)
     assert d["rho"] == 0.0 and d["u"] == 0.0 and d["f"] == 0.0, d
+
+
+def test_python_asl_layer_locomotive_style(tmp_path):
+    """The Python mirror of ASL's names in a locomotive-style driver:
+    readSurface(file, Block), LBGKTurbulence on D3Q15, a velocity map tunnel,
+    a no-slip STL body, pressure-velocity ends and the surface force; against
+    the oracle fed the same maps, bit for bit."""
+    import asl_amd
+    dx = 0.25
+    grid = asl_amd.Block((40, 18, 22), dx, (-dx, -dx, -dx))
+    path = tmp_path / "body.stl"
+    # the body's shell stays cells away from the tunnel's: a velocity-map ghost
+    # whose average would take in a body ghost is rejected (ASL would read that
+    # ghost's stale populations)
+    write_stl_binary(path, icosphere(0.6, (2.5, 1.4, 3.0)))
+    body = asl_amd.readSurface(path, grid)
+    size = tuple(s - 2 for s in grid.size)
+    # tunnel as the drivers build it (not normalised): inside of a cylinder along x,
+    # capped half a cell beyond the block's ends, cut by the ground plane y = 0
+    pos = geometry.positions(size, dx, (0.0, 0.0, 0.0))
+    r = (size[2] - 1) * dx / 2.1
+    cx, cy, cz = 0.5 * (size[0] - 1) * dx, 0.25 * r, 0.5 * (size[2] - 1) * dx
+    half = 0.5 * ((size[0] - 1) * dx + dx)
+    tunnel = (r - np.sqrt((pos[1] - np.float32(cy)) ** 2 + (pos[2] - np.float32(cz)) ** 2)).astype(np.float32)
+    tunnel = np.minimum(np.minimum(tunnel, np.float32(half) - np.abs(pos[0] - np.float32(cx))), pos[1])
+    tunnel = tunnel.astype(np.float32)
+    nu, u0 = 0.002, 0.1
+    lb = asl_amd.LBGKTurbulence(asl_amd.Block(size), nu, asl_amd.d3q15())
+    lb.init()
+    asl_amd.LBGKUtilities(lb).initF((u0, 0.0, 0.0))
+    bcs = [asl_amd.generateBCVelocity(lb, (u0, 0.0, 0.0), tunnel),
+           asl_amd.generateBCNoSlip(lb, body),
+           asl_amd.generateBCConstantPressureVelocity(lb, 1.0, (u0, 0.0, 0.0), ["X0", "XE"])]
+    asl_amd.initAll(bcs)
+    asl_amd.executeAll(bcs)
+    force = asl_amd.generateComputeSurfaceForce(lb, body)
+    force.init()
+    for _ in range(60):
+        lb.execute()
+        asl_amd.executeAll(bcs)
+    force.execute()
+    O = oracle.OracleLattice(size, 15, nu=nu)
+    O.set_turbulence(True)
+    O.init_f(1.0, (u0, 0, 0))
+    ids = [O.add_map_bc(O.VELOCITY_MAP, tunnel, u=(u0, 0, 0)), O.add_map_bc(O.NOSLIP_MAP, body),
+           O.add_slice_bc(O.CONSTANT_PRESSURE_VELOCITY, _lib.X0 | _lib.XE, p=1.0, u=(u0, 0, 0))]
+    for b in ids:
+        O.execute_bc(b)
+    O.run(60, ids)
+    ct = lb.lattice.cell_types()
+    comp = (ct == _lib.CODE_FLUID) | (ct == _lib.CODE_FLUID_BC)
+    assert comp.sum() > 1000 and (body <= 0).sum() > 50
+    assert np.array_equal(lb.getRho()[comp], O.rho()[comp].astype(np.float32))
+    assert np.array_equal(lb.getVelocity()[:, comp], O.velocity()[:, comp].astype(np.float32))
+    inner = (slice(None),) + (slice(1, -1),) * 3
+    fo = O.surface_force(body)[inner]
+    assert np.abs(fo).max() > 1e-3
+    assert np.array_equal(force.field[inner], fo.astype(np.float32))
