@@ -375,6 +375,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline sample")
+    ap.add_argument("--remap", type=int, default=-1, metavar="G",
+                    help="XCD run length of the step kernel's chunks (default: the library's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES bytes device to device once and exit (reference kernel of known "
                          "traffic for the HBM counters)")
@@ -416,6 +418,9 @@ def main():
 
     mapv, u = setup_fields()
     L = build_gpu(mapv, u)
+    if args.remap >= 0:
+        from asl_amd import _lib
+        L.set_tuning(_lib.TUNE_XCD_REMAP, args.remap)
     st = L.stats()
     counted = st["counted"]
     interior = int(np.prod(SIZE))
