@@ -240,6 +240,15 @@ __device__ __forceinline__ int xcd_remap(int L, int n, int g) {
   return grp * span + (r & 7) * g + (r >> 3);
 }
 
+// Workgroup b of n taking the chunks in reverse order within its XCD's share:
+// the chunks b' with b' % 8 == b % 8 are visited from the last to the first,
+// so every chunk stays on the XCD (and in the L2) that processed it in the
+// previous step.
+__device__ __forceinline__ int xcd_reverse(int b, int n) {
+  const int r = b & 7, nr = (n - r + 7) >> 3;
+  return r + 8 * (nr - 1 - (b >> 3));
+}
+
 __device__ __forceinline__ int wrap1(int v, int n) { return v < 1 ? n : (v > n ? 1 : v); }
 
 template <class C>
@@ -450,16 +459,15 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L, void *
 // out of them and spills into vector registers (D3Q19 FP32: 103 VGPRs
 // instead of 70).  Loads through a plain pointer stay where they are used.
 //
-// Odd steps take the chunks in reverse order (zig-zag over steps): a step
-// then starts where the previous one ended, on the populations written last,
-// which are the ones still held in the 256 MiB Infinity Cache.  Every cell's
-// reads and writes in one step are its own (AA pattern), so the order of the
-// chunks does not change results.  Measured on the 256x128x128 channel:
-// 0.1327 -> 0.1281 ms per step.
+// Odd steps take the chunks in reverse order (zig-zag over steps), each XCD
+// over its own share (xcd_reverse): a step then starts where the previous
+// one ended, on the populations written last, which its XCD's L2 still
+// holds.  Every cell's reads and writes in one step are its own (AA
+// pattern), so the order of the chunks does not change results.
 template <int Q, class S, bool ODD, bool PER, bool WIDE, bool CL>
 __global__ __launch_bounds__(STEP_BLOCK) void k_step(const StepArgs *args) {
   const StepArgs &a = *args;
-  const int b = ODD ? a.nblocks - 1 - static_cast<int>(blockIdx.x) : static_cast<int>(blockIdx.x);
+  const int b = ODD ? xcd_reverse(static_cast<int>(blockIdx.x), a.nblocks) : static_cast<int>(blockIdx.x);
   step_body<Q, S, ODD, PER, WIDE, CL>(a, a.remap > 1 ? xcd_remap(b, a.nblocks, a.remap) : b);
 }
 
@@ -528,7 +536,8 @@ __global__ __launch_bounds__(BLOCK) void k_surface(const SurfaceArgs a) {
 // Odd steps take the entries in reverse order, as k_step does.
 template <int Q, class S, bool ODD, bool PER, bool WIDE>
 __global__ __launch_bounds__(STEP_BLOCK) void k_step_batch(const StepArgs *desc, const uint2 *ent) {
-  const uint2 e = ent[ODD ? gridDim.x - 1 - blockIdx.x : blockIdx.x];
+  const int b = static_cast<int>(blockIdx.x);
+  const uint2 e = ent[ODD ? xcd_reverse(b, static_cast<int>(gridDim.x)) : b];
   step_body<Q, S, ODD, PER, WIDE>(desc[e.x], static_cast<int>(e.y));
 }
 

@@ -163,7 +163,7 @@ struct aslhip_lattice {
   bool deferred_first = false;
   // launch tuning
   int tune_block = STEP_BLOCK;
-  int tune_remap = 0;  // chunk runs per XCD (0: hardware order; the autotuner may choose others)
+  int tune_remap = 4;  // chunk runs per XCD (measured best for graph chunks; the autotuner may choose others)
   // face rule: per face slot * 4 + mode of the winning slice BC, and values
   int fface[6] = {0, 0, 0, 0, 0, 0};
   float fval32[6][27] = {};
