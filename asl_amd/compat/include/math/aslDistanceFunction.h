@@ -1,10 +1,13 @@
 // aslDistanceFunction.h — level-set geometry (src/math/aslDistanceFunction.h,
-// src/math/aslDistanceFunction.cxx:55-164), evaluated on the host in the
-// element type of the map (float for FP32 maps), with ASL's expression order.
+// src/math/aslDistanceFunction.cxx:55-164).  A tree emits itself as a postfix
+// program (emit) that aslhip_df_map evaluates on the GPU at every node of a
+// map; getDistance evaluates the same expression at one point on the host,
+// in the element type of the map, with ASL's expression order.
 #ifndef ASLDISTANCEFUNCTION_H
 #define ASLDISTANCEFUNCTION_H
 
 #include <memory>
+#include <vector>
 
 #include "aslVectors.h"
 
@@ -15,6 +18,8 @@ class DistanceFunction {
   virtual ~DistanceFunction() {}
   virtual float getDistance(const float *pos) const = 0;
   virtual double getDistance(const double *pos) const = 0;
+  // appends the postfix program of this expression (ASLHIP_DF_* op codes)
+  virtual void emit(std::vector<double> &prog) const = 0;
 };
 typedef std::shared_ptr<DistanceFunction> SPDistanceFunction;
 

@@ -1,13 +1,19 @@
 // Level-set geometry of the compat layer: ASL's distance functions
 // (src/math/aslDistanceFunction.cxx:55-164) and generators
-// (src/aslGeomInc.cxx:33-221), evaluated on the host in the element type of
-// the map (constants rounded to that type, as in ASL's generated kernels).
+// (src/aslGeomInc.cxx:33-221).  Maps are built on the GPU from the tree's
+// postfix program (aslhip_df_map); getDistance is the same expression at one
+// point on the host (constants rounded to the element type, as in ASL's
+// generated kernels).
 #include <algorithm>
 #include <cmath>
 
 #include <aslGeomInc.h>
+#include <aslhip.h>
 
 namespace asl {
+namespace detail {
+void check_abi(int r);
+}
 namespace {
 
 template <class Impl>
@@ -28,6 +34,7 @@ class Sphere : public DFBase<Sphere> {
     const T l2 = (d0 * d0 + d1 * d1) + d2 * d2;
     return std::sqrt(l2) - T(r_);
   }
+  void emit(std::vector<double> &p) const override { p.insert(p.end(), {ASLHIP_DF_SPHERE, c_[0], c_[1], c_[2], r_}); }
 
  private:
   double r_, c_[3];
@@ -49,6 +56,10 @@ class Cylinder : public DFBase<Cylinder> {
     const T rad = std::sqrt((e0 * e0 + e1 * e1) + e2 * e2);
     return cone_ ? rad - T(tan_) * proj : rad - T(r_);
   }
+  void emit(std::vector<double> &p) const override {
+    p.insert(p.end(), {double(cone_ ? ASLHIP_DF_CONE : ASLHIP_DF_CYLINDER), c_[0], c_[1], c_[2], o_[0], o_[1], o_[2],
+                       cone_ ? tan_ : r_});
+  }
 
  private:
   double r_;
@@ -67,6 +78,7 @@ class Plane : public DFBase<Plane> {
   T eval(const T *p) const {
     return ((p[0] * T(n_[0]) + p[1] * T(n_[1])) + p[2] * T(n_[2])) - T(b_);
   }
+  void emit(std::vector<double> &p) const override { p.insert(p.end(), {ASLHIP_DF_PLANE, n_[0], n_[1], n_[2], b_}); }
 
  private:
   double n_[3], b_;
@@ -80,6 +92,11 @@ class Binary : public DFBase<Binary> {
     const T x = a_->getDistance(p), y = b_->getDistance(p);
     return and_ ? std::max(x, y) : std::min(x, y);
   }
+  void emit(std::vector<double> &p) const override {
+    a_->emit(p);
+    b_->emit(p);
+    p.push_back(and_ ? ASLHIP_DF_AND : ASLHIP_DF_OR);
+  }
 
  private:
   SPDistanceFunction a_, b_;
@@ -91,6 +108,10 @@ class Inversion : public DFBase<Inversion> {
   explicit Inversion(SPDistanceFunction a) : a_(a) {}
   template <class T>
   T eval(const T *p) const { return -a_->getDistance(p); }
+  void emit(std::vector<double> &p) const override {
+    a_->emit(p);
+    p.push_back(ASLHIP_DF_NEG);
+  }
 
  private:
   SPDistanceFunction a_;
@@ -106,27 +127,32 @@ class Normalization : public DFBase<Normalization> {
     const T top = dist > T(1.) ? T(1.) : dist;
     return dist < T(-1.) ? T(-1.) : top;
   }
+  void emit(std::vector<double> &p) const override {
+    a_->emit(p);
+    p.insert(p.end(), {ASLHIP_DF_NORMALIZE, f_});
+  }
 
  private:
   SPDistanceFunction a_;
   double f_;
 };
 
-template <class T>
+// The map of df on the container's ghost-inclusive block, on the GPU.
 void sample(AbstractData &d, const DistanceFunction &df) {
   const Block &b = d.getBlock();
-  const AVec<int> &S = b.getSize();
+  if (nD(b) != 3) errorMessage("initData: a 3D block is expected");
+  std::vector<double> prog;
+  df.emit(prog);
+  const int size[3] = {b.getSize()[0], b.getSize()[1], b.getSize()[2]};
+  const double pos[3] = {b.position[0], b.position[1], b.position[2]};
   std::vector<double> &out = d.values_.at(0);
-  const T p0[3] = {T(b.position[0]), T(b.position[1]), T(b.position[2])};
-  const T dx = T(b.dx);
-  long j = 0;
-  for (int x = 0; x < S[0]; ++x)
-    for (int y = 0; y < S[1]; ++y)
-      for (int z = 0; z < S[2]; ++z, ++j) {
-        // Index2PositionACL: position + dx * index (src/math/aslIndex2Position.cxx:64-108)
-        const T pos[3] = {p0[0] + dx * T(x), p0[1] + dx * T(y), p0[2] + dx * T(z)};
-        out[j] = double(df.getDistance(pos));
-      }
+  if (d.getTypeID() == acl::TYPE_DOUBLE) {
+    detail::check_abi(aslhip_df_map(prog.data(), int(prog.size()), size, pos, b.dx, 1, out.data()));
+  } else {
+    std::vector<float> m(out.size());
+    detail::check_abi(aslhip_df_map(prog.data(), int(prog.size()), size, pos, b.dx, 0, m.data()));
+    std::copy(m.begin(), m.end(), out.begin());
+  }
 }
 
 }  // namespace
@@ -217,10 +243,7 @@ SPDistanceFunction normalize(SPDistanceFunction a, double dx) {
 
 void initData(SPAbstractDataWithGhostNodes d, SPDistanceFunction f) {
   if (!f) errorMessage("DistanceFunction::initData - argument is not initialized");
-  if (d->getTypeID() == acl::TYPE_DOUBLE)
-    sample<double>(*d, *f);
-  else
-    sample<float>(*d, *f);
+  sample(*d, *f);
 }
 
 SPAbstractDataWithGhostNodes generateDataContainer_SP(const Block &b, SPDistanceFunction df, unsigned int gN,

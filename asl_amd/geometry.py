@@ -144,3 +144,74 @@ def read_surface(path, size, dx, position=(0.0, 0.0, 0.0)):
     tri = stl_to_asl_axes(read_stl(path))
     pos, h = block_nodes(size, dx, position)
     return surface_map(tri, size, pos, h, 3.6 * dx, 1.2 * dx, dx, 1.8)
+
+
+# --- distance-function expressions evaluated on the GPU (initData(map, DF)) --
+# Postfix programs of the ops in include/aslhip.h (ASLHIP_DF_*), built like
+# the generators of src/aslGeomInc.cxx:33-221; df_map samples one on a grid.
+
+def _unit(v):
+    v = np.asarray(v, dtype=np.float64)
+    return v / np.sqrt((v * v).sum())
+
+
+def df_sphere(r, c):
+    return [1.0, float(c[0]), float(c[1]), float(c[2]), float(r)]
+
+
+def df_cylinder_inf(r, axis, c):
+    o = _unit(axis)
+    return [2.0, float(c[0]), float(c[1]), float(c[2]), o[0], o[1], o[2], float(r)]
+
+
+def df_plane(n, p0):
+    """DistFPlane(n, p0): p . n/|n| - (p0 . n)/|n| (src/math/aslDistanceFunction.cxx:133-143)."""
+    n = np.asarray(n, dtype=np.float64)
+    o = _unit(n)
+    return [4.0, o[0], o[1], o[2], float(np.dot(np.asarray(p0, dtype=np.float64), n) / np.sqrt((n * n).sum()))]
+
+
+def df_and(a, b):
+    return list(a) + list(b) + [5.0]
+
+
+def df_or(a, b):
+    return list(a) + list(b) + [6.0]
+
+
+def df_neg(a):
+    return list(a) + [7.0]
+
+
+def df_normalize(a, dx=1.0):
+    return list(a) + [8.0, 1.0 / dx / 1.8]
+
+
+def df_cylinder(r, l, c):
+    """generateDFCylinder: the infinite cylinder cut by the planes at c +- l/2."""
+    l, c = np.asarray(l, dtype=np.float64), np.asarray(c, dtype=np.float64)
+    return df_and(df_and(df_cylinder_inf(r, l, c), df_plane(l, c + 0.5 * l)), df_plane(-l, c - 0.5 * l))
+
+
+def df_cone(r, l, a):
+    """generateDFCone(r, l, a): DistFCone(r/|l|, l, a) cut at the apex and the base."""
+    l, a = np.asarray(l, dtype=np.float64), np.asarray(a, dtype=np.float64)
+    o = _unit(l)
+    cone = [3.0, a[0], a[1], a[2], o[0], o[1], o[2], float(np.tan(r / np.sqrt((l * l).sum())))]
+    return df_and(df_and(cone, df_plane(l, a + l)), df_plane(-l, a))
+
+
+def df_map(prog, shape, pos, dx, fp64=False):
+    """The map of a program on the nodes pos + dx * index of a grid of ``shape``
+    (on the GPU; see ``aslhip_df_map``), float32 (or float64) of that shape."""
+    import ctypes
+
+    from . import _lib
+    pr = np.ascontiguousarray(np.asarray(prog, dtype=np.float64).reshape(-1))
+    sz = np.ascontiguousarray(np.asarray(shape, dtype=np.int32).reshape(3))
+    pa = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(3))
+    out = np.empty(tuple(int(s) for s in sz), dtype=np.float64 if fp64 else np.float32)
+    dp = ctypes.POINTER(ctypes.c_double)
+    _lib.call("aslhip_df_map", pr.ctypes.data_as(dp), pr.size, sz.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+              pa.ctypes.data_as(dp), float(dx), int(bool(fp64)), out.ctypes.data)
+    return out

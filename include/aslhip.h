@@ -195,6 +195,33 @@ int aslhip_synchronize(aslhip_lattice *h);
 int aslhip_surface_force(aslhip_lattice *h, const float *map, void *out);
 int aslhip_surface_flux(aslhip_lattice *h, const float *map, void *out);
 
+/* asl::initData(map, DF) / generateDataContainer_SP(block, DF, gN, type) —
+ * src/aslGeomInc.cxx:187-221 with the DistanceFunction tree of
+ * src/math/aslDistanceFunction.cxx:55-164: the map of a distance-function
+ * expression on a grid of size[3] nodes, node (i, j, k) at pos + dx * (i, j, k)
+ * evaluated in the element type (fp64 ? double : float; constants rounded to
+ * it where they are used, every operation in ASL's order, rounded once).
+ * prog: nwords doubles in postfix order, each op code followed by its
+ * parameters:
+ *   SPHERE c0 c1 c2 r            sqrt(l2(p - c)) - r
+ *   CYLINDER c0 c1 c2 o0 o1 o2 r  sqrt(l2(d - (d.o) o)) - r, d = p - c, |o| = 1
+ *   CONE a0 a1 a2 o0 o1 o2 t      sqrt(l2(d - (d.o) o)) - t (d.o), d = p - a
+ *   PLANE n0 n1 n2 b              p.n - b
+ *   AND / OR                      max / min of the top two values
+ *   NEG                           -x
+ *   NORMALIZE f                   clamp(f x, -1, 1)
+ * out: host array of size[0] * size[1] * size[2] elements, last index fastest. */
+#define ASLHIP_DF_SPHERE 1
+#define ASLHIP_DF_CYLINDER 2
+#define ASLHIP_DF_CONE 3
+#define ASLHIP_DF_PLANE 4
+#define ASLHIP_DF_AND 5
+#define ASLHIP_DF_OR 6
+#define ASLHIP_DF_NEG 7
+#define ASLHIP_DF_NORMALIZE 8
+int aslhip_df_map(const double *prog, int nwords, const int size[3], const double pos[3], double dx, int fp64,
+                  void *out);
+
 /* asl::readSurface(file, ...) — src/readers/aslVTKFormatReaders.h:66-78,
  * src/readers/aslVTKFormatReaders.cxx:266-305: the triangles of a binary or
  * ASCII STL file, 9 doubles each (3 vertices, the file's x y z), rounded to

@@ -6,4 +6,4 @@ leg may import this package.  It wraps ``liblbgk_oracle.so`` (built from
 ``asl::LBGK`` pool/window streaming, collision and LBGK boundary conditions
 (see the header of ``lbgk_oracle.cpp`` for the ASL files each part follows).
 """
-from .lbgk import OracleLattice, load_library, surface_distance_map, template, triangle_d2  # noqa: F401
+from .lbgk import OracleLattice, df_map, load_library, surface_distance_map, template, triangle_d2  # noqa: F401

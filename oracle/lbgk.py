@@ -35,6 +35,8 @@ def load_library():
     lib.orc_set_compressible.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_set_omega.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     lib.orc_set_turbulence.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_df_map.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                               ctypes.c_int, ctypes.c_void_p]
     lib.orc_triangle_d2.restype = ctypes.c_double
     lib.orc_triangle_d2.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     lib.orc_surface_distance_map.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
@@ -88,6 +90,17 @@ def template(q):
     if n < 0:
         raise ValueError("unknown template %r" % (q,))
     return (np.array(c[: 3 * n], dtype=np.int64).reshape(n, 3), np.array(w[:n]), np.array(inv[:n], dtype=np.int64))
+
+
+def df_map(prog, shape, pos, dx, fp64=False):
+    """Distance-function map of a postfix program on shape nodes at pos + dx * index."""
+    pr = np.ascontiguousarray(np.asarray(prog, dtype=np.float64).reshape(-1))
+    sz = np.ascontiguousarray(np.asarray(shape, dtype=np.int32).reshape(3))
+    pa = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(3))
+    out = np.empty(tuple(int(s) for s in sz), dtype=np.float64 if fp64 else np.float32)
+    load_library().orc_df_map(pr.ctypes.data, pr.size, sz.ctypes.data, pa.ctypes.data, float(dx), int(bool(fp64)),
+                              out.ctypes.data)
+    return out
 
 
 def triangle_d2(p, tri):

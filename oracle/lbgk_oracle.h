@@ -71,6 +71,10 @@ int orc_template(int q, int *c, double *w, int *inv);
  * and readSurface's map of ntri triangles (9 doubles each) on a grid of
  * size[3] nodes at pos + h * index: map = ((sqrt(min(cap^2, d^2)) - shift) / div1) / div2. */
 double orc_triangle_d2(const double p[3], const double t[9]);
+/* Distance-function map (df_map.cpp) of a postfix program (include/aslhip.h's
+ * ASLHIP_DF_* op codes) on size[3] nodes at pos + dx * index, in float or double. */
+void orc_df_map(const double *prog, int nwords, const int size[3], const double pos[3], double dx, int fp64,
+                void *out);
 void orc_surface_distance_map(const double *tri, long long ntri, const int size[3], const double pos[3],
                               const double h[3], double cap, double shift, double div1, double div2, float *map);
 
