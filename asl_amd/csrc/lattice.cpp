@@ -162,7 +162,7 @@ struct aslhip_lattice {
   bool deferred_raw = false;
   bool deferred_first = false;
   // launch tuning
-  int tune_block = STEP_BLOCK;
+  int tune_block = 512;   // workgroup size (measured best with runs of 4 chunks per XCD)
   int tune_remap = 4;  // chunk runs per XCD (measured best for graph chunks; the autotuner may choose others)
   // face rule: per face slot * 4 + mode of the winning slice BC, and values
   int fface[6] = {0, 0, 0, 0, 0, 0};
@@ -1298,7 +1298,7 @@ int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_re
   const int blocks[3] = {256, 512, 1024};
   double best = 1e300;
   int bb = h->tune_block, br = h->tune_remap;
-  for (int remap : {0, 4, 16, 64})
+  for (int remap : {0, 2, 4, 8, 16, 64})
     for (int blk : blocks) {
       if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_XCD_REMAP, remap)) return r;
       if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_BLOCK, blk)) return r;

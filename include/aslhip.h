@@ -275,7 +275,7 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
  *   consecutive chunks (neighbouring chunks share one L2; default 4); 0 or
  *   1 = the hardware round-robin order.
  *   ASLHIP_TUNE_BLOCK: workgroup size, a multiple of 64 up to 1024 (default
- *   1024); a workgroup takes that many consecutive cells of a plane. */
+ *   512); a workgroup takes that many consecutive cells of a plane. */
 #define ASLHIP_TUNE_XCD_REMAP 1
 #define ASLHIP_TUNE_BLOCK 2
 /* Batches only: graph chunks step groups of consecutive lattices of at most
@@ -285,7 +285,7 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
 int aslhip_set_tuning(aslhip_lattice *h, int knob, int value);
 
 /* The autotuner: times nsteps (rounded up to even) real steps of the lattice
- * for every candidate setting (XCD runs 0/4/16/64 x workgroup 256/512/1024) with
+ * for every candidate setting (XCD runs 0/2/4/8/16/64 x workgroup 256/512/1024) with
  * HIP events, keeps the fastest and returns it (either pointer may be NULL).
  * The trial steps advance the simulation like any other steps. */
 int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_remap);

@@ -94,11 +94,11 @@ def test_launch_tuning_and_autotune_keep_results():
         L.step(70, graph=True)
         L.step(50)
         same(L, ref)
-    ref.step(120)
+    ref.step(180)
     L, _ = cavity()
-    best = L.autotune(10)  # 12 candidates x 10 real steps
-    assert best[0] in (256, 512, 1024) and best[1] in (0, 4, 16, 64)
-    L.step(240 - 120)
+    best = L.autotune(10)  # 18 candidates x 10 real steps
+    assert best[0] in (256, 512, 1024) and best[1] in (0, 2, 4, 8, 16, 64)
+    L.step(300 - 180)
     same(L, ref)
     with pytest.raises(asl_amd.ASLError):
         L.set_tuning(_lib.TUNE_BLOCK, 100)
