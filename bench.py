@@ -9,7 +9,9 @@ no-slip side walls; initial u = (0.04, 0, 0) + 1e-3 * U[-1, 1] per cell and
 component from a seeded generator, u = 0 inside the sphere.  A step is one
 fused collide+stream pass over the whole lattice with every BC applied.
 
-The timed region holds exactly --steps steps (hipGraph replays of 100-step
+Before the warm-up the library's autotuner times every workgroup size and
+XCD run length over 20 real steps each and keeps the fastest (reported under
+roofline.launch; --no-autotune keeps the defaults).  The timed region holds exactly --steps steps (hipGraph replays of 100-step
 chunks) with the populations already in HBM; it is bracketed by stream
 synchronisations.  ``value`` is MLUPS over computation nodes (solid and
 ghost cells are not counted) from the wall clock around it.  The roofline
@@ -19,7 +21,7 @@ up to 1000 eager launches on the lattice's stream.
 ``cpu_baseline`` times the CPU oracle (the C++ restatement of ASL's LBGK) on
 a bounded sample of the same lattice with the host's cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-autotune] [--remap G]
        python bench.py --copy-calibration BYTES   (one plain device copy, for
        scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters)
        python bench.py --precision-study N [--snapshots 500,2000,5000]
@@ -200,6 +202,7 @@ def cavity_study(no_cpu=False, steps=1000):
     elsewhere), FP32: GPU MLUPS and step-kernel bandwidth, CPU oracle MLUPS."""
     n = 64
     L = batch_cavity(n, 0.6)
+    block, remap = L.autotune(20)
     L.step(100, write_macro=False, graph=True)
     L.synchronize()
     t0 = time.perf_counter()
@@ -212,7 +215,8 @@ def cavity_study(no_cpu=False, steps=1000):
     line = {"study": "64^3 lid-driven cavity, D3Q19 FP32", "steps": steps,
             "gpu_mlups": round(cells * steps / wall / 1e6, 1), "ms_per_step": round(wall * 1e3 / steps, 5),
             "kernel_ms_avg": round(kern_s * 1e3, 5),
-            "kernel_gbps": round((152.0 * cells + cells) / kern_s / 1e9, 1)}
+            "kernel_gbps": round((152.0 * cells + cells) / kern_s / 1e9, 1),
+            "launch": {"workgroup": block, "xcd_run": remap}}
     if not no_cpu:
         import oracle
         from oracle.lbgk import set_threads
@@ -376,7 +380,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline sample")
     ap.add_argument("--remap", type=int, default=-1, metavar="G",
-                    help="XCD run length of the step kernel's chunks (default: the library's)")
+                    help="XCD run length of the step kernel's chunks (default: the autotuner's choice)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch settings instead of autotuning them")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES bytes device to device once and exit (reference kernel of known "
                          "traffic for the HBM counters)")
@@ -418,9 +424,13 @@ def main():
 
     mapv, u = setup_fields()
     L = build_gpu(mapv, u)
+    from asl_amd import _lib
+    # the library's autotuner picks the workgroup size and XCD run length from
+    # timed real steps (they advance the flow like the warm-up steps)
+    block, remap = L.autotune(20) if not args.no_autotune else (None, None)
     if args.remap >= 0:
-        from asl_amd import _lib
         L.set_tuning(_lib.TUNE_XCD_REMAP, args.remap)
+        remap = args.remap
     st = L.stats()
     counted = st["counted"]
     interior = int(np.prod(SIZE))
@@ -463,7 +473,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel_ms_avg": round(kern_s * 1e3, 5), "kernel_launches_timed": nk,
-                     "bytes_per_launch": bytes_per_launch, "graph_ms_per_step": round(ms_graph / args.steps, 5)},
+                     "bytes_per_launch": bytes_per_launch, "graph_ms_per_step": round(ms_graph / args.steps, 5),
+                     "launch": {"workgroup": block, "xcd_run": remap}},
         "cpu_baseline": None,
     }
     if not args.no_cpu:
