@@ -58,6 +58,7 @@ SIGNATURES = {
     "aslhip_surface_flux": [_vp, _fp, _vp],
     "aslhip_template": [ctypes.c_int, _ip, _dp, _ip],
     "aslhip_batch_autotune_groups": [_vp, ctypes.c_int, _ip],
+    "aslhip_get_tuning": [_vp, _ip, _ip],
     "aslhip_df_map": [_dp, ctypes.c_int, _ip, _dp, ctypes.c_double, ctypes.c_int, _vp],
     "aslhip_stl_count": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong)],
     "aslhip_stl_read": [ctypes.c_char_p, _dp, ctypes.c_longlong],

@@ -1273,6 +1273,13 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]) {
   return 0;
 }
 
+int aslhip_get_tuning(aslhip_lattice *h, int *block, int *remap) {
+  if (int r = check(h)) return r;
+  if (block) *block = h->tune_block;
+  if (remap) *remap = h->tune_remap;
+  return 0;
+}
+
 int aslhip_set_tuning(aslhip_lattice *h, int knob, int value) {
   if (int r = check(h)) return r;
   if (knob == ASLHIP_TUNE_XCD_REMAP) {

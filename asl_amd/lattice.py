@@ -153,6 +153,12 @@ class Lattice:
     def set_tuning(self, knob, value):
         call("aslhip_set_tuning", self.h, int(knob), int(value))
 
+    def tuning(self):
+        """The current launch settings: (workgroup size, XCD run length)."""
+        b, r = ctypes.c_int(0), ctypes.c_int(0)
+        call("aslhip_get_tuning", self.h, ctypes.byref(b), ctypes.byref(r))
+        return b.value, r.value
+
     def autotune(self, nsteps=20):
         """Time every launch setting over nsteps real steps and keep the fastest: (block, remap)."""
         b, r = ctypes.c_int(0), ctypes.c_int(0)

@@ -9,9 +9,11 @@ no-slip side walls; initial u = (0.04, 0, 0) + 1e-3 * U[-1, 1] per cell and
 component from a seeded generator, u = 0 inside the sphere.  A step is one
 fused collide+stream pass over the whole lattice with every BC applied.
 
-Before the warm-up the library's autotuner times every workgroup size and
-XCD run length over 20 real steps each and keeps the fastest (reported under
-roofline.launch; --no-autotune keeps the defaults).  The timed region holds exactly --steps steps (hipGraph replays of 100-step
+The step kernel runs with the library's default launch settings (512-cell
+workgroups, runs of 4 chunks per XCD; reported under roofline.launch), so
+that the rocprofv3 summaries of the same command time the same launches;
+--autotune lets the library's autotuner choose them first (every workgroup
+size and XCD run length timed over 20 real steps).  The timed region holds exactly --steps steps (hipGraph replays of 100-step
 chunks) with the populations already in HBM; it is bracketed by stream
 synchronisations.  ``value`` is MLUPS over computation nodes (solid and
 ghost cells are not counted) from the wall clock around it.  The roofline
@@ -21,7 +23,7 @@ up to 1000 eager launches on the lattice's stream.
 ``cpu_baseline`` times the CPU oracle (the C++ restatement of ASL's LBGK) on
 a bounded sample of the same lattice with the host's cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-autotune] [--remap G]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--autotune] [--remap G]
        python bench.py --copy-calibration BYTES   (one plain device copy, for
        scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters)
        python bench.py --precision-study N [--snapshots 500,2000,5000]
@@ -380,9 +382,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline sample")
     ap.add_argument("--remap", type=int, default=-1, metavar="G",
-                    help="XCD run length of the step kernel's chunks (default: the autotuner's choice)")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the library's default launch settings instead of autotuning them")
+                    help="XCD run length of the step kernel's chunks (default: the library's)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the library's autotuner choose the launch settings before timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES bytes device to device once and exit (reference kernel of known "
                          "traffic for the HBM counters)")
@@ -427,7 +429,7 @@ def main():
     from asl_amd import _lib
     # the library's autotuner picks the workgroup size and XCD run length from
     # timed real steps (they advance the flow like the warm-up steps)
-    block, remap = L.autotune(20) if not args.no_autotune else (None, None)
+    block, remap = L.autotune(20) if args.autotune else L.tuning()
     if args.remap >= 0:
         L.set_tuning(_lib.TUNE_XCD_REMAP, args.remap)
         remap = args.remap

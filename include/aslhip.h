@@ -283,6 +283,8 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
  * step), so that a group can stay in the 256 MiB Infinity Cache. */
 #define ASLHIP_TUNE_GROUP_MB 3
 int aslhip_set_tuning(aslhip_lattice *h, int knob, int value);
+/* The lattice's current workgroup size and XCD run length (either may be NULL). */
+int aslhip_get_tuning(aslhip_lattice *h, int *block, int *remap);
 
 /* The autotuner: times nsteps (rounded up to even) real steps of the lattice
  * for every candidate setting (XCD runs 0/2/4/8/16/64 x workgroup 256/512/1024) with

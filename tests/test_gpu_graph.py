@@ -96,8 +96,10 @@ def test_launch_tuning_and_autotune_keep_results():
         same(L, ref)
     ref.step(180)
     L, _ = cavity()
+    assert L.tuning() == (512, 4)  # the defaults
     best = L.autotune(10)  # 18 candidates x 10 real steps
     assert best[0] in (256, 512, 1024) and best[1] in (0, 2, 4, 8, 16, 64)
+    assert L.tuning() == best
     L.step(300 - 180)
     same(L, ref)
     with pytest.raises(asl_amd.ASLError):
