@@ -19,12 +19,53 @@ namespace aslhip {
 constexpr int CL_NIN = 7, CL_H = 32, CL_NPARAM = 1345;
 constexpr int CL_W1 = 0, CL_B1 = 224, CL_W2 = 256, CL_B2 = 1280, CL_W3 = 1312, CL_B3 = 1344;
 
-// Closure parameters as the step kernel reads them (device memory).
+// Closure parameters as the step kernel reads them (device memory).  The
+// matrix-core path reads its operands pre-arranged in lane order (filled by
+// pack_closure from p): one 8-byte load per BF16 A operand and 16-byte loads
+// for the biases and output weights, instead of one load and one conversion
+// per element.
 struct ClosureParams {
   float p[CL_NPARAM];  // BF16-rounded weights and biases
   float mu[CL_NIN], sc[CL_NIN];
   float out;           // scale of the relaxation-time correction
+  alignas(16) unsigned long long a1[64];  // lane l: W1'[l % 32][4 (l / 32) + j], j < 4 (BF16 x 4)
+  unsigned long long a2[4][64];           // k-block kb, lane l: W2[l % 32][8 kb + 4 (l / 32) + j]
+  alignas(16) float b2[2][16];            // half h, register r: b2[8 (r / 4) + 4 h + r % 4]
+  alignas(16) float w3[2][16];            // half h, register r: w3[8 (r / 4) + 4 h + r % 4]
 };
+
+// BF16 bits of an already BF16-rounded float (the top half of its bits).
+inline unsigned long long bf16x4_pack(const float *v) {
+  unsigned long long r = 0;
+  for (int j = 0; j < 4; ++j) {
+    unsigned u;
+    __builtin_memcpy(&u, &v[j], 4);
+    r |= static_cast<unsigned long long>(u >> 16) << (16 * j);
+  }
+  return r;
+}
+
+// Arranges the matrix-core operands of c.p (already BF16-rounded).
+inline void pack_closure(ClosureParams &c) {
+  for (int l = 0; l < 64; ++l) {
+    const int col = l & 31, half = l >> 5;
+    float v[4];
+    for (int j = 0; j < 4; ++j) {
+      const int k = 4 * half + j;
+      v[j] = k < CL_NIN ? c.p[CL_W1 + col * CL_NIN + k] : c.p[CL_B1 + col];
+    }
+    c.a1[l] = bf16x4_pack(v);
+    for (int kb = 0; kb < 4; ++kb) {
+      for (int j = 0; j < 4; ++j) v[j] = c.p[CL_W2 + col * CL_H + 8 * kb + 4 * half + j];
+      c.a2[kb][l] = bf16x4_pack(v);
+    }
+  }
+  for (int h = 0; h < 2; ++h)
+    for (int r = 0; r < 16; ++r) {
+      c.b2[h][r] = c.p[CL_B2 + 8 * (r / 4) + 4 * h + (r % 4)];
+      c.w3[h][r] = c.p[CL_W3 + 8 * (r / 4) + 4 * h + (r % 4)];
+    }
+}
 
 template <int Q>
 __device__ __forceinline__ void closure_features(const float (&f)[Q], float r, const float (&m)[3], float (&x)[7]) {
@@ -80,7 +121,7 @@ typedef float f32x16_t __attribute__((ext_vector_type(16)));
 // operand needs for k-block r / 4, so layers chain without shuffles.
 __device__ __forceinline__ float closure_y_mfma(const float (&x)[7], const ClosureParams &c) {
   const int l = static_cast<int>(threadIdx.x) & 63;
-  const int col = l & 31, half = l >> 5;
+  const int half = l >> 5;
   unsigned short own[8];
 #pragma unroll
   for (int j = 0; j < 7; ++j) own[j] = bf16_bits((x[j] - c.mu[j]) * c.sc[j]);
@@ -97,14 +138,11 @@ __device__ __forceinline__ float closure_y_mfma(const float (&x)[7], const Closu
     par[2] = static_cast<unsigned short>(phi & 0xffffu);
     par[3] = static_cast<unsigned short>(phi >> 16);
   }
-  // weights (BF16-rounded floats, so the top halves are exact)
-  bf16x4_t a1;
+  // weights in lane order (pack_closure)
+  bf16x4_t a1, a2[4];
+  __builtin_memcpy(&a1, &c.a1[l], 8);
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int k = 4 * half + j;
-    const float wv = k < 7 ? c.p[CL_W1 + col * 7 + k] : c.p[CL_B1 + col];
-    a1[j] = static_cast<short>(bf16_bits(wv));
-  }
+  for (int kb = 0; kb < 4; ++kb) __builtin_memcpy(&a2[kb], &c.a2[kb][l], 8);
   float y[2];
 #pragma unroll
   for (int cb = 0; cb < 2; ++cb) {
@@ -116,24 +154,22 @@ __device__ __forceinline__ float closure_y_mfma(const float (&x)[7], const Closu
     f32x16_t d1 = {};
     d1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, b1, d1, 0, 0, 0);
     f32x16_t d2;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) d2[r] = c.p[CL_B2 + 8 * (r / 4) + 4 * half + (r % 4)];
+    __builtin_memcpy(&d2, c.b2[half], 64);
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
-      bf16x4_t a2, b2;
+      bf16x4_t b2;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        a2[j] = static_cast<short>(bf16_bits(c.p[CL_W2 + col * CL_H + 8 * kb + 4 * half + j]));
         const float h = d1[4 * kb + j];
         b2[j] = static_cast<short>(bf16_bits(h > 0.f ? h : 0.f));
       }
-      d2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, b2, d2, 0, 0, 0);
+      d2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2[kb], b2, d2, 0, 0, 0);
     }
     float part = 0.f;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const float h = d2[r] > 0.f ? d2[r] : 0.f;
-      part = __builtin_fmaf(c.p[CL_W3 + 8 * (r / 4) + 4 * half + (r % 4)], h, part);
+      part = __builtin_fmaf(c.w3[half][r], h, part);
     }
     const float other = __shfl_xor(part, 32);
     y[cb] = (half == 0 ? part + other : other + part) + c.p[CL_B3];

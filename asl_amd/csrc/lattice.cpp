@@ -899,6 +899,7 @@ int aslhip_closure_set(aslhip_lattice *h, const float *params, const float mu[7]
     c.sc[j] = sc[j];
   }
   c.out = out_scale;
+  pack_closure(c);
   if (!h->d_cl) HIPCHK(hipMalloc(&h->d_cl, sizeof(ClosureParams)));
   HIPCHK(hipStreamSynchronize(h->stream));  // steps queued earlier read the old parameters
   HIPCHK(hipMemcpy(h->d_cl, &c, sizeof c, hipMemcpyHostToDevice));
