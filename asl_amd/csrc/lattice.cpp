@@ -182,7 +182,9 @@ int check(aslhip_lattice *h) {
   return 0;
 }
 
+// Destroys the captured chunks once the stream has finished any replay of them.
 void drop_graphs(aslhip_lattice *h) {
+  if ((h->graph[0] || h->graph[1]) && h->stream) (void)hipStreamSynchronize(h->stream);
   for (auto &g : h->graph)
     if (g) {
       (void)hipGraphExecDestroy(g);
@@ -664,7 +666,9 @@ int build_graph(aslhip_lattice *h, bool odd) {
   if (g) {
     if (h->graph_fl[odd ? 1 : 0] == fl) return 0;
     // captured with other flags (frame, compressibility, relaxation model):
-    // its launches name the old kernel variant and argument slot
+    // its launches name the old kernel variant and argument slot; a replay of
+    // it may still be in flight
+    HIPCHK(hipStreamSynchronize(h->stream));
     (void)hipGraphExecDestroy(g);
     g = nullptr;
   }
@@ -1431,12 +1435,13 @@ int batch_tables(aslhip_batch *b) {
     for (int c = 0; c < a.nblocks; ++c) ent.push_back(make_uint2(static_cast<unsigned>(l), static_cast<unsigned>(c)));
   }
   b->groups.emplace_back(static_cast<int>(gfirst), static_cast<int>(ent.size()));
+  // replays in flight read the old tables and graphs: finish them first
+  HIPCHK(hipStreamSynchronize(b->stream));
   for (auto &g : b->graph)
     if (g) {
       (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
-  HIPCHK(hipStreamSynchronize(b->stream));
   if (b->ent) HIPCHK(hipFree(b->ent));
   b->ent = nullptr;
   HIPCHK(hipMalloc(&b->ent, ent.size() * sizeof(uint2)));

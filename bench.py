@@ -261,7 +261,7 @@ def batch_cavity(n, tau, lid=0.05):
     return L
 
 
-def batch_study(steps, warmup):
+def batch_study(steps, warmup, group_mb=None):
     """MLUPS of 64 uneven cavities: one batched launch per step (hipGraph chunks)
     against the same lattices stepped one launch per lattice and step."""
     import asl_amd
@@ -271,7 +271,11 @@ def batch_study(steps, warmup):
     B = asl_amd.Batch(lat)
     B.step(max(warmup, 1), write_macro=False)
     block = B.autotune(10)
-    group_mb = B.autotune_groups(2)
+    if group_mb is None:
+        group_mb = B.autotune_groups(2)
+    else:
+        from asl_amd import _lib
+        B.set_tuning(_lib.TUNE_GROUP_MB, group_mb)
     ms_batch = B.time_steps(steps, graph=True)
     solo = [batch_cavity(n, t) for n, t in specs]
     for L in solo:
@@ -399,6 +403,8 @@ def main():
     ap.add_argument("--eval-chunks", type=int, default=16)
     ap.add_argument("--cavity-study", action="store_true",
                     help="only measure the 64^3 lid-driven cavity (GPU and CPU oracle)")
+    ap.add_argument("--group-mb", type=int, default=-1, metavar="MB",
+                    help="batch study: lattice-group size instead of the group autotuner's choice")
     ap.add_argument("--batch-study", action="store_true",
                     help="only run the batched-lattices study (64 cavities of uneven sizes)")
     args = ap.parse_args()
@@ -409,7 +415,8 @@ def main():
         closure_study(args.closure_study, args.tau_fine, args.chunk, args.train_snapshots, args.eval_chunks)
         return
     if args.batch_study:
-        batch_study(args.steps if args.steps != 5000 else 1000, args.warmup if args.warmup != 200 else 20)
+        batch_study(args.steps if args.steps != 5000 else 1000, args.warmup if args.warmup != 200 else 20,
+                    args.group_mb if args.group_mb >= 0 else None)
         return
     if args.precision_study:
         precision_study(args.precision_study, [int(x) for x in args.snapshots.split(",")])
