@@ -33,6 +33,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--autotun
            [--eval-chunks E]  (learned closure: N^3 coarse vs (4N)^3 fine decaying
            turbulence; KE(t) of fine, coarse and coarse+closure, wall times)
        python bench.py --cavity-study   (the 64^3 lid-driven cavity: GPU and CPU-oracle MLUPS)
+       python bench.py --storage-study  (the sphere channel with FP32, FP16, BF16, FP64 storage)
        python bench.py --batch-study [--steps K]
            (64 lid-driven cavities of log-uniform sizes 32..160 and tau in
            [0.51, 1]: one batched launch per step vs one launch per lattice)
@@ -110,10 +111,10 @@ def tgv_fields(n, u0=0.04):
     return rho, u
 
 
-def build_gpu(mapv, u):
+def build_gpu(mapv, u, storage="f32"):
     import asl_amd
     from asl_amd import _lib
-    L = asl_amd.Lattice(SIZE, 19, dtype="f32", nu=NU)
+    L = asl_amd.Lattice(SIZE, 19, dtype=storage, nu=NU)
     L.init_equilibrium(1.0, u)
     bcs = [L.add_slice_bc(_lib.BC_CONSTANT_VELOCITY, _lib.X0, u=(U_IN, 0.0, 0.0)),
            L.add_slice_bc(_lib.BC_CONSTANT_PRESSURE, _lib.XE, p=1.0),
@@ -238,6 +239,28 @@ def cavity_study(no_cpu=False, steps=1000):
         line["cpu_cores"] = threads
         line["cpu_sample_steps"] = k
     print(json.dumps(line), flush=True)
+
+
+def storage_study(steps=2000):
+    """The sphere channel with each population storage type: step-kernel time,
+    hipGraph MLUPS and algorithmic bandwidth (FP32 collision throughout; FP64
+    for the f64 lattice; BF16/FP16 store f - 1)."""
+    mapv, u = setup_fields()
+    for storage, bpc in (("f32", 152.0), ("f16", 76.0), ("bf16", 76.0), ("f64", 304.0)):
+        L = build_gpu(mapv, u, storage)
+        counted = L.stats()["counted"]
+        L.step(200, write_macro=False, graph=True)
+        L.synchronize()
+        ms_graph, _ = L.time_steps(steps, write_macro=False, graph=True)
+        _, ms_kernel = L.time_steps(500, write_macro=False)
+        kern = ms_kernel / 500
+        byts = bpc * counted + 1.0 * int(np.prod(SIZE))
+        print(json.dumps({"study": "population storage, 256x128x128 sphere channel", "storage": storage,
+                          "bytes_per_cell_update": bpc, "kernel_ms": round(kern, 5),
+                          "graph_mlups": round(counted * steps / ms_graph / 1e3, 1),
+                          "kernel_tbps_algorithmic": round(byts / (kern / 1e3) / 1e12, 3),
+                          "frac_of_8tbps": round(byts / (kern / 1e3) / 8e12, 4)}), flush=True)
+        del L
 
 
 def batch_specs(n=64, seed=SEED):
@@ -403,6 +426,8 @@ def main():
     ap.add_argument("--eval-chunks", type=int, default=16)
     ap.add_argument("--cavity-study", action="store_true",
                     help="only measure the 64^3 lid-driven cavity (GPU and CPU oracle)")
+    ap.add_argument("--storage-study", action="store_true",
+                    help="the sphere channel with FP32, FP16, BF16 and FP64 population storage")
     ap.add_argument("--group-mb", type=int, default=-1, metavar="MB",
                     help="batch study: lattice-group size instead of the group autotuner's choice")
     ap.add_argument("--batch-study", action="store_true",
@@ -410,6 +435,9 @@ def main():
     args = ap.parse_args()
     if args.cavity_study:
         cavity_study(args.no_cpu)
+        return
+    if args.storage_study:
+        storage_study()
         return
     if args.closure_study:
         closure_study(args.closure_study, args.tau_fine, args.chunk, args.train_snapshots, args.eval_chunks)
