@@ -39,3 +39,32 @@ def test_storage_precision_ranks_against_fp64():
     assert err["f32"][0] < err["f16"][0] < err["bf16"][0]
     assert err["f16"][0] < 2e-3 and err["bf16"][0] < 2e-2
     assert err["f16"][1] < 1e-3 and err["bf16"][1] < 1e-2
+
+
+def bf16_rne(x):
+    """Round float32 values to BF16 (nearest, ties to even), as float32 (numpy)."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "f16"])
+def test_storage_rounding_is_nearest_even(dtype):
+    """write_f stores f - 1 rounded to nearest, ties to even (FP16: the IEEE
+    conversion; BF16: the hardware conversion), read_f returns 1 + that."""
+    size = (10, 9, 8)
+    S = tuple(s + 2 for s in size)
+    rng = np.random.default_rng(4)
+    f = (1.0 + rng.standard_normal((19,) + S) * 10.0 ** rng.uniform(-6, -1, (19,) + S)).astype(np.float32)
+    if dtype == "bf16":
+        # exact BF16 ties in f - 1 (a BF16 value plus half its last place), where 1 + tie is exact
+        dev = (f[:6] - np.float32(1.0)).astype(np.float32)
+        tie = (bf16_rne(dev).view(np.uint32) | np.uint32(0x8000)).view(np.float32)
+        ok = (np.abs(tie) > 1e-4) & ((np.float32(1.0) + tie) - np.float32(1.0) == tie)
+        f[:6] = np.where(ok, np.float32(1.0) + tie, f[:6])
+        assert ok.sum() > 1000
+    L = asl_amd.Lattice(size, 19, dtype=dtype, nu=0.05)
+    L.write_f(f)
+    dev = (f - np.float32(1.0)).astype(np.float32)  # exact: f lies in [1/2, 2]
+    stored = bf16_rne(dev) if dtype == "bf16" else dev.astype(np.float16).astype(np.float32)
+    assert np.array_equal(L.f(), np.float32(1.0) + stored)
