@@ -98,11 +98,13 @@ __device__ __forceinline__ float closure_relax(float y, const ClosureParams &c, 
   return 1.f / (tau < lo ? lo : tau);
 }
 
-// Round to the nearest BF16 (ties to even); the bits of the result.
+// Round to the nearest BF16 (ties to even); the bits of the result.  gfx950
+// converts in hardware (v_cvt_pk_bf16_f32, two values per instruction).
 __device__ __forceinline__ unsigned short bf16_bits(float x) {
-  unsigned u = __float_as_uint(x);
-  if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<unsigned short>(u >> 16);
+  const __bf16 b = static_cast<__bf16>(x);
+  unsigned short u;
+  __builtin_memcpy(&u, &b, 2);
+  return u;
 }
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
