@@ -557,16 +557,15 @@ __global__ __launch_bounds__(256) void k_closure_features(const StepArgs *args, 
   float v[Q];
 #pragma unroll
   for (int k = 0; k < Q; ++k) {
-    unsigned o;
+    // each direction from its own base (a.qs32 is 0 on lattices over 4 GiB)
     if (ODD) {
       const unsigned src = static_cast<unsigned>(wrap1(x - Lat<Q>::c[k][0], a.nx) * a.sx +
                                                  wrap1(y - Lat<Q>::c[k][1], a.ny) * a.sy +
                                                  wrap1(z - Lat<Q>::c[k][2], a.nz));
-      o = static_cast<unsigned>(inv_dir<Q>(k)) * a.qs32 + ES * src;
+      v[k] = CD::ld(a.fd[inv_dir<Q>(k)], ES * src);
     } else {
-      o = static_cast<unsigned>(k) * a.qs32 + ES * idx;
+      v[k] = CD::ld(a.fd[k], ES * idx);
     }
-    v[k] = CD::ld(a.fd[0], o);
   }
   float r, m[3], f7[7];
   moments<Q, float>(v, r, m);

@@ -163,6 +163,7 @@ struct aslhip_lattice {
   bool deferred_first = false;
   // launch tuning
   int tune_block = 512;   // workgroup size (measured best with runs of 4 chunks per XCD)
+  bool force_wide = false;  // 64-bit addressing on any size (ASLHIP_TUNE_WIDE)
   int tune_remap = 4;  // chunk runs per XCD (measured best for graph chunks; the autotuner may choose others)
   // face rule: per face slot * 4 + mode of the winning slice BC, and values
   int fface[6] = {0, 0, 0, 0, 0, 0};
@@ -567,7 +568,7 @@ int run_vlinks(aslhip_lattice *h, bool after_odd) {
   return 0;
 }
 
-bool is_wide(const aslhip_lattice *h) { return h->qs_bytes * h->Q > 0xffffffffLL; }
+bool is_wide(const aslhip_lattice *h) { return h->force_wide || h->qs_bytes * h->Q > 0xffffffffLL; }
 
 // Kernel arguments of one step of h (flags filled in by the caller).
 StepArgs step_args(const aslhip_lattice *h, int block = 0) {
@@ -1287,6 +1288,13 @@ int aslhip_get_tuning(aslhip_lattice *h, int *block, int *remap) {
 
 int aslhip_set_tuning(aslhip_lattice *h, int knob, int value) {
   if (int r = check(h)) return r;
+  if (knob == ASLHIP_TUNE_WIDE) {
+    if (value != 0 && value != 1) return fail(ASLHIP_EINVAL, "ASL ERROR: ASLHIP_TUNE_WIDE takes 0 or 1");
+    if (int r = flush_deferred(h)) return r;
+    if ((value != 0) != h->force_wide) drop_graphs(h);  // captured launches name the variant
+    h->force_wide = value != 0;
+    return 0;
+  }
   if (knob == ASLHIP_TUNE_XCD_REMAP) {
     if (value < 0 || value > 4096) return fail(ASLHIP_EINVAL, "ASL ERROR: XCD run length %d outside 0..4096", value);
     h->tune_remap = value;

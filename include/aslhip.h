@@ -282,6 +282,10 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
  * this many MiB of populations one group at a time (0: all lattices each
  * step), so that a group can stay in the 256 MiB Infinity Cache. */
 #define ASLHIP_TUNE_GROUP_MB 3
+/* Lattices: 1 = address the populations with 64-bit offsets (the variant
+ * that lattices over 4 GiB of populations use) whatever the size; 0 = by
+ * size (default).  For testing that variant on small lattices. */
+#define ASLHIP_TUNE_WIDE 4
 int aslhip_set_tuning(aslhip_lattice *h, int knob, int value);
 /* The lattice's current workgroup size and XCD run length (either may be NULL). */
 int aslhip_get_tuning(aslhip_lattice *h, int *block, int *remap);
