@@ -1,0 +1,88 @@
+"""The level-set geometry of ASL's flow drivers flow2.cc and flow3.cc
+(examples/flow), rebuilt as distance-function programs (include/aslhip.h's
+ASLHIP_DF_* postfix code) with the compat layer's double-precision vector
+arithmetic in the same order, so that the maps equal the ones the drivers
+build -- checked against the maps the drivers write -- and give the oracle the
+ghost-layer values the written output leaves out."""
+import math
+
+from asl_amd import geometry as g
+
+
+def _add(a, b):
+    return [a[0] + b[0], a[1] + b[1], a[2] + b[2]]
+
+
+def _sub(a, b):
+    return [a[0] - b[0], a[1] - b[1], a[2] - b[2]]
+
+
+def _scale(a, s):
+    return [a[0] * s, a[1] * s, a[2] * s]
+
+
+def _dot(a, b):
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+
+
+def _cross(a, b):
+    return [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+
+
+def plane(n, p0):
+    s = math.sqrt(_dot(n, n))
+    return [4.0, n[0] / s, n[1] / s, n[2] / s, _dot(p0, n) / s]
+
+
+def cylinder_inf(r, l, c):
+    s = math.sqrt(_dot(l, l))
+    return [2.0, c[0], c[1], c[2], l[0] / s, l[1] / s, l[2] / s, r]
+
+
+def cylinder(r, l, c):
+    return g.df_and(g.df_and(cylinder_inf(r, l, c), plane(l, _add(c, _scale(l, 0.5)))),
+                    plane(_scale(l, -1.0), _sub(c, _scale(l, 0.5))))
+
+
+def prism(points):
+    center = list(points[0])
+    for p in points[1:]:
+        center = _add(center, p)
+    center = [v / float(len(points)) for v in center]
+    axis = _cross(_sub(points[0], center), _sub(points[1], center))
+    res = plane(_cross(_sub(points[1], points[0]), axis), points[0])
+    for i in range(1, len(points)):
+        res = g.df_and(res, plane(_cross(_sub(points[(i + 1) % len(points)], points[i]), axis), points[i]))
+    return res
+
+
+def flow2_mirror(x, y):
+    h, r, a, b = 10.0, 1.5, 10.0, 6.0
+    rect = [_scale([.866 * a, -b, -a * .5], .5), _scale([-.866 * a, -b, a * .5], .5),
+            _scale([-.866 * a, b, a * .5], .5), _scale([.866 * a, b, -a * .5], .5)]
+    r1c = [x, y + r * .5 + b * .5, h]
+    r2c = [x, y - r * .5 - b * .5, h]
+    res = g.df_or(prism([_add(r1c, p) for p in rect]), prism([_add(r2c, p) for p in rect]))
+    res = g.df_and(g.df_and(res, plane([.5, 0., .866], _add(r1c, [r, 0., 0.]))),
+                   plane([-.5, 0., -.866], _sub(r1c, [r, 0., 0.])))
+    return g.df_or(res, cylinder(r, [0., 0., h], [x, y, h * .5]))
+
+
+def flow2_mirrors():
+    res = None
+    for i in range(8):
+        for j in range(10):
+            m = flow2_mirror(25. + 20. * i, 25. + 15. * j)
+            res = m if res is None else g.df_or(res, m)
+    return res
+
+
+def flow3_cylinders(size=(50, 50, 50)):
+    r, spacing = 2.0, 4.0
+    res = None
+    for i in range(int(math.ceil(size[0] / (2 * r + spacing)))):
+        for j in range(int(math.ceil(size[1] / (2 * r + spacing)))):
+            c = cylinder_inf(r, [0., 0., 1.], [i * (2. * r + spacing) + r + spacing / 2.,
+                                               j * (2. * r + spacing) + r + spacing / 2., 0.])
+            res = c if res is None else g.df_or(res, c)
+    return res

@@ -11,6 +11,8 @@ compatibility headers (skipped where the ASL source tree is not mounted).
 import os
 import subprocess
 
+import numpy as np
+
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -71,3 +73,72 @@ def test_asl_flow_drivers_compile_unchanged(driver, tmp_path):
            "-L", os.path.join(ROOT, "asl_amd"), "-laslcompat", "-laslhip"]
     p = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
     assert p.returncode == 0, p.stderr[-2000:]
+
+
+def _compare_driver_output(vti_path, size, q, nu, bcs, prog, steps):
+    """The driver's output against the oracle run of the same set-up.  The map
+    comes from the driver's own geometry rebuilt as a distance-function program
+    (the written map covers the internal cells only; the ghost layer matters
+    for the BCs) and must equal the written map bit for bit first."""
+    import numpy as np
+
+    import oracle
+    from vti import read_vti
+    out = read_vti(vti_path)
+    S = tuple(s + 2 for s in size)
+    full = oracle.df_map(prog, S, (-1.0, -1.0, -1.0), 1.0)
+    inner = (slice(1, -1),) * 3
+    assert np.array_equal(full[inner], out["map"])
+    O = oracle.OracleLattice(size, q, fp64=False, nu=nu)
+    O.init_f(1.0, (0.0, 0.0, 0.0))
+    ids = []
+    for kind, faces, p, u in bcs:
+        ids.append(O.add_map_bc(O.NOSLIP_MAP, full) if kind == "map" else O.add_slice_bc(kind, faces, p=p, u=u))
+    for b in ids:
+        O.execute_bc(b)
+    O.run(steps, ids)
+    fluid = out["map"] > 0
+    rho_o = O.rho()[inner].astype(np.float32)
+    v_o = O.velocity()[(slice(None),) + inner].astype(np.float32)
+    assert fluid.sum() > 1000
+    assert np.array_equal(out["rho"][fluid], rho_o[fluid])
+    assert np.array_equal(out["v"][:, fluid], v_o[:, fluid])
+    return np.abs(v_o[:, fluid]).max()
+
+
+@pytest.mark.gpu
+def test_asl_flow2_driver_unchanged(tmp_path):
+    """ASL's examples/flow/flow2.cc, built unchanged against the compat layer,
+    run on the GPU: its 11th output (901 steps) equals the oracle run of the
+    same set-up bit for bit on the fluid cells."""
+    from asl_driver_geometry import flow2_mirrors
+    from oracle import OracleLattice as OL
+    exe = os.path.join(BIN, "asl_flow2")
+    if not os.path.exists(exe):
+        pytest.skip("ASL's flow drivers were not built (ASL source tree not mounted at build time)")
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=600, cwd=tmp_path)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    umax = _compare_driver_output(tmp_path / "flow2_10.vti", (200, 200, 25), 15, float(np.float32(0.0125)),
+                                  [(OL.CONSTANT_PRESSURE, 1 | 2, 1.0, None),
+                                   (OL.CONSTANT_VELOCITY, 32, 0.0, (0.1, 0.0, 0.0)),
+                                   (OL.NOSLIP, 4 | 8 | 16, 0.0, None), ("map", None, 0.0, None)],
+                                  flow2_mirrors(), 901)
+    assert umax > 0.01
+
+
+@pytest.mark.gpu
+def test_asl_flow3_driver_unchanged(tmp_path):
+    """ASL's examples/flow/flow3.cc (pressure-driven flow through a cylinder
+    array, BCConstantPressure objects with addSliceX0/XE) likewise."""
+    from asl_driver_geometry import flow3_cylinders
+    from oracle import OracleLattice as OL
+    exe = os.path.join(BIN, "asl_flow3")
+    if not os.path.exists(exe):
+        pytest.skip("ASL's flow drivers were not built (ASL source tree not mounted at build time)")
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=600, cwd=tmp_path)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    umax = _compare_driver_output(tmp_path / "flow3Res_10.vti", (50, 50, 50), 15, float(np.float32(0.00625)),
+                                  [(OL.CONSTANT_PRESSURE, 1, 1.2, None), (OL.CONSTANT_PRESSURE, 2, 0.8, None),
+                                   (OL.NOSLIP, 4 | 8 | 16 | 32, 0.0, None), ("map", None, 0.0, None)],
+                                  flow3_cylinders(), 901)
+    assert umax > 0.01
