@@ -86,3 +86,29 @@ def flow3_cylinders(size=(50, 50, 50)):
                                                j * (2. * r + spacing) + r + spacing / 2., 0.])
             res = c if res is None else g.df_or(res, c)
     return res
+
+
+def in_block(size, dx, position, ng):
+    """generateDFInBlock(b, nG): the union of the six planes (nG - 1/2) dx inside the faces."""
+    offs = (ng - .5) * dx
+    p0 = [position[0] + offs, position[1] + offs, position[2] + offs]
+    pe = [position[d] + (size[d] - 1) * dx - offs for d in range(3)]
+    res = None
+    for d in range(3):
+        nc = [0., 0., 0.]
+        nc[d] = 1.
+        part = g.df_or(plane(nc, p0), plane(_scale(nc, -1.0), pe))
+        res = part if res is None else g.df_or(res, part)
+    return res
+
+
+def rotating_cylinders(size=(100, 100, 150), dx=1.0):
+    """flowRotatingCylinders.cc: outer cylinder map, inner cylinder map, computational domain."""
+    gs = [dx * s for s in size]
+    c = [.5 * v for v in gs]
+    ex = g.df_normalize(g.df_neg(cylinder_inf(.48 * gs[0], [0., 0., 1.], c)), dx)
+    inner = g.df_normalize(cylinder_inf(.24 * gs[0], [0., 0., 1.], c), dx)
+    dom = g.df_normalize(g.df_or(g.df_or(g.df_neg(cylinder_inf(.48 * gs[0], [0., 0., 1.], c)),
+                                         cylinder_inf(.24 * gs[0], [0., 0., 1.], c)),
+                                 in_block(size, dx, (0., 0., 0.), 0)), dx)
+    return ex, inner, dom, c

@@ -142,3 +142,40 @@ def test_asl_flow3_driver_unchanged(tmp_path):
                                    (OL.NOSLIP, 4 | 8 | 16 | 32, 0.0, None), ("map", None, 0.0, None)],
                                   flow3_cylinders(), 901)
     assert umax > 0.01
+
+
+@pytest.mark.gpu
+def test_asl_flow_rotating_cylinders_driver_unchanged(tmp_path):
+    """ASL's examples/flow/flowRotatingCylinders.cc (Couette flow between
+    counter-rotating cylinders: two velocity-map BCs with rotation fields and
+    a computational-domain map), built unchanged: its second output (1001
+    steps) equals the oracle bit for bit on the computational domain."""
+    from asl_driver_geometry import rotating_cylinders
+    from vti import read_vti
+
+    import oracle
+    exe = os.path.join(BIN, "asl_flowRotatingCylinders")
+    if not os.path.exists(exe):
+        pytest.skip("ASL's flow drivers were not built (ASL source tree not mounted at build time)")
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=600, cwd=tmp_path)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    out = read_vti(tmp_path / "flowRotCylRes_2.vti")
+    size, w = (100, 100, 150), 2e-3
+    ex, inner_cyl, dom, c = rotating_cylinders(size)
+    S = tuple(s + 2 for s in size)
+    maps = [oracle.df_map(prg, S, (-1.0, -1.0, -1.0), 1.0) for prg in (ex, inner_cyl, dom)]
+    inner = (slice(1, -1),) * 3
+    assert np.array_equal(maps[2][inner], out["map"])
+    O = oracle.OracleLattice(size, 15, fp64=False, nu=float(np.float32(0.01)))
+    O.init_f(1.0, (0.0, 0.0, 0.0))
+    ids = [O.add_map_bc(O.VELOCITY_MAP, maps[0], cdomain=maps[2], u=(0, 0, 0), rot=(0, 0, w) + tuple(c)),
+           O.add_map_bc(O.VELOCITY_MAP, maps[1], cdomain=maps[2], u=(0, 0, 0), rot=(0, 0, -2 * w) + tuple(c)),
+           O.add_slice_bc(O.NOSLIP, 16 | 32)]
+    for b in ids:
+        O.execute_bc(b)
+    O.run(1001, ids)
+    fluid = (maps[0][inner] > 0) & (maps[1][inner] > 0) & (out["map"] > 0)
+    v_o = O.velocity()[(slice(None),) + inner].astype(np.float32)
+    assert fluid.sum() > 100000 and np.abs(v_o[:, fluid]).max() > 0.01
+    assert np.array_equal(out["rho"][fluid], O.rho()[inner].astype(np.float32)[fluid])
+    assert np.array_equal(out["v"][:, fluid], v_o[:, fluid])
