@@ -41,6 +41,7 @@ SIGNATURES = {
     "aslhip_set_viscosity": [_vp, ctypes.c_double],
     "aslhip_set_compressible": [_vp, ctypes.c_int],
     "aslhip_set_omega": [_vp, _dp],
+    "aslhip_set_geometry": [_vp, ctypes.c_double, _dp],
     "aslhip_set_turbulence": [_vp, ctypes.c_int],
     "aslhip_set_compute_macro": [_vp, ctypes.c_int, ctypes.c_int],
     "aslhip_init_equilibrium": [_vp, ctypes.c_double, _dp],

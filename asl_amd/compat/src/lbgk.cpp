@@ -86,6 +86,9 @@ void LBGK::create(const Block &b, acl::TypeID t) {
   type_ = t;
   check_abi(aslhip_lattice_create(&h_, b.getSize()[0], b.getSize()[1], b.getSize()[2], q,
                                   t == acl::TYPE_DOUBLE ? ASLHIP_F64 : ASLHIP_F32, 0));
+  // positions of the rotating velocity-map BCs: the block's dx and position
+  const double pos[3] = {b.position[0], b.position[1], b.position[2]};
+  check_abi(aslhip_set_geometry(h_, b.dx, pos));
   rho_ = std::make_shared<DataWithGhostNodesACLData>(b, 1, t);
   vel_ = std::make_shared<DataWithGhostNodesACLData>(b, 3, t);
   f_ = std::make_shared<DataWithGhostNodesACLData>(b, q, t);

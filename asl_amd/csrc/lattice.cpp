@@ -882,6 +882,19 @@ int aslhip_set_omega(aslhip_lattice *h, const double omega[3]) {
   return 0;
 }
 
+int aslhip_set_geometry(aslhip_lattice *h, double dx, const double position[3]) {
+  if (int r = check(h)) return r;
+  if (!position) return fail(ASLHIP_EINVAL, "ASL ERROR: set_geometry: null position");
+  if (!(dx > 0.) || !std::isfinite(dx) ||
+      !(std::isfinite(position[0]) && std::isfinite(position[1]) && std::isfinite(position[2])))
+    return fail(ASLHIP_EINVAL, "ASL ERROR: set_geometry: bad dx or position");
+  if (int r = flush_deferred(h)) return r;  // a pending step's BCs use the old positions
+  h->dx = dx;
+  for (int d = 0; d < 3; ++d) h->pos0[d] = position[d];
+  h->dirty = true;  // captured graphs hold the velocity-link arguments by value
+  return 0;
+}
+
 int aslhip_closure_set(aslhip_lattice *h, const float *params, const float mu[7], const float sc[7], float out_scale) {
   if (int r = check(h)) return r;
   if (params && (h->Q != 19 || h->dtype == ASLHIP_F64))

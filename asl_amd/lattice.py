@@ -101,6 +101,11 @@ class Lattice:
         else:
             call("aslhip_set_omega", self.h, (ctypes.c_double * 3)(*[float(x) for x in omega]))
 
+    def set_geometry(self, dx=1.0, position=(0.0, 0.0, 0.0)):
+        """The asl::Block the lattice was built on: dx and the internal block's
+        position, where rotating velocity-map BCs evaluate omega x (r - centre)."""
+        call("aslhip_set_geometry", self.h, float(dx), (ctypes.c_double * 3)(*[float(x) for x in position]))
+
     def set_compute_macro(self, rho=True, vel=True):
         call("aslhip_set_compute_macro", self.h, int(bool(rho)), int(bool(vel)))
 
@@ -401,6 +406,7 @@ class LBGK(NumMethod):
         self.vectorTemplate = vt
         self.lattice = Lattice(block.size, vt.q, dtype=dtype, periodic=periodic, nu=nu)
         self.lattice.set_compute_macro(compRho, compVel)
+        self.lattice.set_geometry(block.dx, block.position)
 
     def init(self):
         pass

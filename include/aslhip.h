@@ -134,6 +134,14 @@ int aslhip_set_turbulence(aslhip_lattice *h, int on);
  * it off. */
 int aslhip_set_omega(aslhip_lattice *h, const double omega[3]);
 
+/* The lattice's asl::Block geometry: the internal block's dx and position
+ * (asl::LBGK(Block b, ...), src/num/aslLBGK.cxx:114-127 and createData; the populations live
+ * on offset(b, 1), whose node I sits at position - dx + dx * I).  Velocity-map
+ * BCs with a rotation field evaluate omega x (r - centre) at these positions
+ * (Index2PositionACL over the populations' block, src/num/aslLBGKBC.cxx:303-305,
+ * src/math/aslIndex2Position.cxx:64-108).  Default: dx 1, position 0. */
+int aslhip_set_geometry(aslhip_lattice *h, double dx, const double position[3]);
+
 /* compRho / compVel constructor flags (src/num/aslLBGK.cxx:210-218). */
 int aslhip_set_compute_macro(aslhip_lattice *h, int comp_rho, int comp_vel);
 

@@ -119,14 +119,16 @@ def block_bpos(position, size, dx):
     return [position[d] + (size[d] - 1) * dx for d in range(3)]
 
 
-def locomotive_tunnel(position, size, dx):
-    """generateTunnel(block) of examples/flow/locomotive.cc on the lattice's internal
-    block: normalize(-(cylinder & ground plane), dx)."""
+def locomotive_tunnel(position, size, dx, rdiv=2.1):
+    """generateTunnel(block) of examples/flow/locomotive.cc (rdiv 2.1) and
+    locomotive_laminar.cc (rdiv 2) on the lattice's internal block:
+    normalize(-(cylinder & ground plane), dx)."""
     bp = block_bpos(position, size, dx)
     l = bp[0] - position[0] + dx
-    r = (bp[2] - position[2]) / 2.1
+    r = (bp[2] - position[2]) / rdiv
     center = [.5 * (bp[d] + position[d]) for d in range(3)]
     center[1] = position[1] + .25 * r
     ground = list(center)
     ground[1] = position[1]
     return g.df_normalize(g.df_neg(g.df_and(cylinder(r, [l, 0., 0.], center), plane([0., -1., 0.], ground))), dx)
+

@@ -181,51 +181,45 @@ def test_asl_flow_rotating_cylinders_driver_unchanged(tmp_path):
     assert np.array_equal(out["v"][:, fluid], v_o[:, fluid])
 
 
-@pytest.mark.gpu
-def test_asl_locomotive_driver_unchanged(tmp_path):
-    """ASL's examples/flow/locomotive.cc built unchanged: an STL body read with
-    readSurface(file, Block), the tunnel level set, LBGKTurbulence on D3Q15,
-    the velocity-map tunnel, visualisation BCs and the surface force; run on
-    the GPU with a box-shaped body and --dx 0.25 --nu 0.01 --iterations 1001
-    (the driver hands ν to the method in lattice units: with its default
-    0.001 a 0.16 m lattice diverges within 1000 steps in the oracle), its output after
-    1000 steps equals the oracle bit for bit (ρ, u on the fluid nodes, the
-    force field on the interior)."""
+def _locomotive_case(tmp_path, name, args, box, extent, gpos, dx, rdiv, nu, turbulence, steps, out_index, mapname):
+    """Runs one of ASL's locomotive drivers, built unchanged, on a box-shaped STL
+    body (box: lo, hi in file coordinates (x, y, z) = ASL (z, y, x)) and checks
+    its output number out_index against the oracle run of the same set-up: the
+    body map (readSurface(file, Block) on the extent / dx grid at gpos), the
+    tunnel level set, ρ and u on the fluid nodes and the surface force on the
+    interior, all bit for bit."""
     from asl_driver_geometry import locomotive_tunnel
     from test_oracle_surface_map import cube_triangles, write_stl_binary
     from vti import read_vti
 
     import oracle
-    exe = os.path.join(BIN, "asl_locomotive")
+    exe = os.path.join(BIN, "asl_" + name)
     if not os.path.exists(exe):
         pytest.skip("ASL's flow drivers were not built (ASL source tree not mounted at build time)")
-    dx = float(np.float32(0.25))     # asl::Parameter<float> dx, widened to the Block's double
-    # the body in file coordinates (x, y, z) = ASL (z, y, x): 10 m long in ASL x
-    tri = cube_triangles(np.array([7.5, 9.8, -15.0]), np.array([10.5, 11.8, -5.0]))
+    tri = cube_triangles(np.array(box[0]), np.array(box[1]))
     write_stl_binary(tmp_path / "body.stl", tri)
-    p = subprocess.run([exe, "--input", str(tmp_path / "body.stl"), "--dx", "0.25", "--nu", "0.01", "--iterations", "1001"],
+    p = subprocess.run([exe, "--input", str(tmp_path / "body.stl")] + args,
                        capture_output=True, text=True, timeout=600, cwd=tmp_path)
     assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
-    out = read_vti(tmp_path / "locomotive_1.vti")
-    grid = [int(v * (1.0 / dx)) for v in (40.0, 10.0, 15.0)]       # the Block given to readSurface
-    gpos = [-30.0, 8.58, 1.53]
+    out = read_vti(tmp_path / ("%s_%d.vti" % (name, out_index)))
+    grid = [int(v * (1.0 / dx)) for v in extent]                    # the Block given to readSurface
     size = tuple(s - 2 for s in grid)                               # the lattice's internal block
     ipos = [v + dx for v in gpos]                                   # offset(Block, -1)
     tpos = [v - dx for v in ipos]                                   # the tunnel container: offset(block, 1)
     t_asl = tri.astype(np.float32).astype(np.float64)[..., ::-1]
     h = [((gpos[d] + (grid[d] - 1) * dx) - gpos[d]) / (grid[d] - 1) for d in range(3)]
     body = oracle.surface_distance_map(t_asl, grid, gpos, h, 3.6 * dx, 1.2 * dx, dx, 1.8)
-    tunnel = oracle.df_map(locomotive_tunnel(ipos, size, dx), grid, tpos, dx)
+    tunnel = oracle.df_map(locomotive_tunnel(ipos, size, dx, rdiv), grid, tpos, dx)
     inner = (slice(1, -1),) * 3
-    assert np.array_equal(body[inner], out["map"]) and np.array_equal(tunnel[inner], out["tunnel"])
-    O = oracle.OracleLattice(size, 15, fp64=False, nu=float(np.float32(0.01)))
-    O.set_turbulence(True)
+    assert np.array_equal(body[inner], out[mapname]) and np.array_equal(tunnel[inner], out["tunnel"])
+    O = oracle.OracleLattice(size, 15, fp64=False, nu=nu)
+    O.set_turbulence(turbulence)
     O.init_f(1.0, (0.1, 0.0, 0.0))
     ids = [O.add_map_bc(O.VELOCITY_MAP, tunnel, u=(0.1, 0.0, 0.0)), O.add_map_bc(O.NOSLIP_MAP, body),
            O.add_slice_bc(O.CONSTANT_PRESSURE_VELOCITY, 1 | 2, p=1.0, u=(0.1, 0.0, 0.0))]
     for b in ids:
         O.execute_bc(b)
-    O.run(1000, ids)
+    O.run(steps, ids)
     fluid = (body[inner] > 0) & (tunnel[inner] > 0)
     assert fluid.sum() > 100000
     assert np.array_equal(out["rho"][fluid], O.rho()[inner].astype(np.float32)[fluid])
@@ -233,3 +227,30 @@ def test_asl_locomotive_driver_unchanged(tmp_path):
     fo = O.surface_force(body)[(slice(None),) + inner].astype(np.float32)
     assert np.abs(fo).max() > 1e-3
     assert np.array_equal(out["force"], fo)
+
+
+@pytest.mark.gpu
+def test_asl_locomotive_driver_unchanged(tmp_path):
+    """ASL's examples/flow/locomotive.cc built unchanged: an STL body read with
+    readSurface(file, Block), the tunnel level set, LBGKTurbulence on D3Q15,
+    the velocity-map tunnel, visualisation BCs and the surface force; run on
+    the GPU with a 10 m box body and --dx 0.25 --nu 0.01 --iterations 1001
+    (the driver hands ν to the method in lattice units: with its default
+    0.001 a 0.16 m lattice diverges within 1000 steps in the oracle), its
+    output after 1000 steps equals the oracle bit for bit."""
+    _locomotive_case(tmp_path, "locomotive", ["--dx", "0.25", "--nu", "0.01", "--iterations", "1001"],
+                     ([7.5, 9.8, -15.0], [10.5, 11.8, -5.0]), (40.0, 10.0, 15.0), [-30.0, 8.58, 1.53],
+                     float(np.float32(0.25)),  # asl::Parameter<float> dx, widened to the Block's double
+                     2.1, float(np.float32(0.01)), True, 1000, 1, "map")
+
+
+@pytest.mark.gpu
+def test_asl_locomotive_laminar_driver_unchanged(tmp_path):
+    """ASL's examples/flow/locomotive_laminar.cc built unchanged (plain LBGK
+    D3Q15, dx 0.5 m, ν = 0.01 m²/s → 0.04 in lattice units, the tunnel radius
+    half the block height): with a 30 m box body its first output after the
+    start (200 steps) equals the oracle bit for bit."""
+    _locomotive_case(tmp_path, "locomotive_laminar", [], ([-7.0, 0.5, 10.0], [-3.0, 4.5, 40.0]),
+                     (150.0, 37.0, 53.2), [-10.0, -1.63, -31.6], 0.5, 2.0, float(np.float32(0.01 / 0.5 / 0.5)),
+                     False, 200, 1, "locomotive")
+

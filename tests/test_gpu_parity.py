@@ -105,15 +105,22 @@ def test_all_bc_kinds(q, compressible):
     assert d["rho"] == 0.0 and d["u"] == 0.0 and d["f"] == 0.0, d
 
 
-@pytest.mark.parametrize("rotating", [False, True])
+@pytest.mark.parametrize("rotating", [False, True, "block"])
 def test_velocity_map(rotating):
-    # BCVelocityMap: moving obstacle surface (constant or rotating wall)
+    # BCVelocityMap: moving obstacle surface (constant or rotating wall); "block":
+    # the rotation field on a lattice whose Block has dx 0.25 at (-3.1, 2.2, 0.7),
+    # so the wall velocity is omega x (r - centre) at physical positions r
     size = (20, 16, 14)
     body = geometry.normalize(geometry.sphere(size, 3.5, (9.0, 7.5, 6.5)))
     G, O = pair(size, 19, nu=0.05)
     G.init_equilibrium(1.0, (0, 0, 0))
     O.init_f(1.0, (0, 0, 0))
     rot = (0.0, 0.0, 0.01, 9.0, 7.5, 6.5) if rotating else None
+    if rotating == "block":
+        dx, pos = 0.25, (-3.1, 2.2, 0.7)
+        G.set_geometry(dx, pos)
+        O.set_geometry(dx, pos)
+        rot = (0.0, 0.01 / dx, 0.02 / dx, pos[0] + 8.0 * dx, pos[1] + 6.5 * dx, pos[2] + 5.5 * dx)
     bcs = [add_bc(G, O, _lib.BC_NOSLIP, 63),
            add_bc(G, O, _lib.BC_VELOCITY_MAP, mapv=body, u=(0.02, 0.0, 0.01), rot=rot)]
     run(G, O, 60, bcs)
