@@ -393,6 +393,34 @@ def closure_study(nc, tau_fine, chunk, snaps, eval_chunks):
           flush=True)
 
 
+def variant_study(n, steps=500):
+    """Step-kernel cost of the per-cell relaxation variants on a periodic n^3
+    D3Q19 FP32 Taylor-Green lattice (tau 0.56): plain LBGK, LBGKTurbulence, the
+    learned closure on the matrix cores (BF16 MFMA) and in FP32 on the vector
+    ALUs (random network, output scale 0.1).  Kernel time from HIP events on
+    the lattice stream; MLUPS from hipGraph-replayed steps."""
+    import asl_amd
+    from asl_amd import _lib, closure
+    rho, u = tgv_fields(n)
+    nu = (0.56 - 0.5) / 3.0
+    for name in ("plain", "turbulence", "closure_mfma", "closure_fp32"):
+        L = asl_amd.Lattice((n, n, n), 19, periodic=True, nu=nu)
+        L.init_equilibrium(rho, u)
+        if name == "turbulence":
+            L.set_turbulence(True)
+        elif name.startswith("closure"):
+            L.set_closure(closure.init_params(1), None, None, 0.1)
+            L.set_closure_mode(_lib.CLOSURE_BF16_MFMA if name == "closure_mfma" else _lib.CLOSURE_FP32)
+        L.step(100, write_macro=False, graph=True)
+        L.synchronize()
+        ms_graph, _ = L.time_steps(steps, write_macro=False, graph=True)
+        _, ms_kernel = L.time_steps(steps, write_macro=False)
+        print(json.dumps({"study": "per-cell relaxation variants, %d^3 periodic D3Q19 FP32" % n, "variant": name,
+                          "kernel_ms": round(ms_kernel / steps, 5),
+                          "graph_mlups": round(n ** 3 * steps / ms_graph / 1e3, 1)}), flush=True)
+        del L
+
+
 def profile_traffic():
     """HBM bytes per launch of the step kernel from the committed rocprofv3 summary, or None."""
     path = os.path.join(ROOT, "profiles", "lbgk_step_d3q19_f32_sphere_256x128x128_traffic.md")
@@ -424,6 +452,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=50, help="coarse steps between snapshots")
     ap.add_argument("--train-snapshots", type=int, default=16)
     ap.add_argument("--eval-chunks", type=int, default=16)
+    ap.add_argument("--variant-study", type=int, default=0, metavar="N",
+                    help="time plain / LBGKTurbulence / closure step kernels on a periodic N^3 lattice")
     ap.add_argument("--cavity-study", action="store_true",
                     help="only measure the 64^3 lid-driven cavity (GPU and CPU oracle)")
     ap.add_argument("--storage-study", action="store_true",
@@ -438,6 +468,9 @@ def main():
         return
     if args.storage_study:
         storage_study()
+        return
+    if args.variant_study:
+        variant_study(args.variant_study)
         return
     if args.closure_study:
         closure_study(args.closure_study, args.tau_fine, args.chunk, args.train_snapshots, args.eval_chunks)
