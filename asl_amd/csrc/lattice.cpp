@@ -1145,6 +1145,16 @@ int aslhip_read_velocity(aslhip_lattice *h, void *out) {
   return copy_out(h, h->u, out, (size_t)h->len * 3 * h->melem);
 }
 
+int aslhip_macro_device(aslhip_lattice *h, const void **rho, const void **u) {
+  if (int r = check(h)) return r;
+  if (!rho && !u) return fail(ASLHIP_EINVAL, "ASL ERROR: macro_device: no output address");
+  if (int r = flush_deferred(h)) return r;
+  HIPCHK(hipStreamSynchronize(h->stream));  // readable from any stream until the next step
+  if (rho) *rho = h->rho;
+  if (u) *u = h->u;
+  return 0;
+}
+
 int aslhip_read_f(aslhip_lattice *h, void *out) {
   if (int r = check(h)) return r;
   if (int r = flush_deferred(h)) return r;

@@ -189,6 +189,13 @@ class Lattice:
         call("aslhip_read_velocity", self.h, out.ctypes.data)
         return out.reshape((3,) + self.S)
 
+    def macro_device(self):
+        """Device addresses (ints) of rho and u, holding what rho() and
+        velocity() would return; valid until the next step on this lattice."""
+        r, u = ctypes.c_void_p(), ctypes.c_void_p()
+        call("aslhip_macro_device", self.h, ctypes.byref(r), ctypes.byref(u))
+        return r.value, u.value
+
     def f(self):
         out = np.empty(self.q * self.len, dtype=self.host_dtype)
         call("aslhip_read_f", self.h, out.ctypes.data)

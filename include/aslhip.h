@@ -263,6 +263,16 @@ int aslhip_write_f(aslhip_lattice *h, const void *f);
  * hold unspecified values (ASL's hold collided ghost data). */
 int aslhip_read_rho(aslhip_lattice *h, void *out);
 int aslhip_read_velocity(aslhip_lattice *h, void *out);
+
+/* Device addresses of rho (one array) and u (three arrays, component-major),
+ * in the ghost-inclusive ASL layout and the element type aslhip_read_rho /
+ * aslhip_read_velocity copy (double for ASLHIP_F64 lattices, float
+ * otherwise), holding what those reads would return: a deferred step runs
+ * first and the lattice's stream is synchronised, so any stream may read
+ * them until the next step on this lattice.  For a solver coupled on the GPU
+ * to lbgk->getVelocity() (examples/flow/multiphase_flow.cc:182-184, which
+ * hands it to asl::generateFDMultiPhase).  rho or u may be NULL. */
+int aslhip_macro_device(aslhip_lattice *h, const void **rho, const void **u);
 int aslhip_read_f(aslhip_lattice *h, void *out);
 
 /* Packed cell-type map (one byte per ghost-inclusive cell):
