@@ -42,7 +42,7 @@ SIGNATURES = {
     "aslhip_set_compressible": [_vp, ctypes.c_int],
     "aslhip_set_omega": [_vp, _dp],
     "aslhip_set_geometry": [_vp, ctypes.c_double, _dp],
-    "aslhip_macro_device": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
+    "aslhip_macro_device": [_vp, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
     "aslhip_set_turbulence": [_vp, ctypes.c_int],
     "aslhip_set_compute_macro": [_vp, ctypes.c_int, ctypes.c_int],
     "aslhip_init_equilibrium": [_vp, ctypes.c_double, _dp],
@@ -72,7 +72,7 @@ SIGNATURES = {
     "aslhip_stats": [_vp, ctypes.POINTER(ctypes.c_int64)],
     "aslhip_set_tuning": [_vp, ctypes.c_int, ctypes.c_int],
     "aslhip_autotune": [_vp, ctypes.c_int, _ip, _ip],
-    "aslhip_time_steps": [_vp, ctypes.c_int, ctypes.c_uint, _dp, _dp],
+    "aslhip_time_steps": [_vp, ctypes.c_int, ctypes.c_uint, _dp, _dp, _dp],
     "aslhip_batch_create": [ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int],
     "aslhip_batch_destroy": [_vp],
     "aslhip_batch_step": [_vp, ctypes.c_int, ctypes.c_uint],

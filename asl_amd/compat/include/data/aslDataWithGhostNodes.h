@@ -33,10 +33,23 @@ class AbstractData {
   void refresh() {
     if (fetch_) fetch_(*this);
   }
-  // internal: storage and the fetch hook
+  // After values_ were written by a reader (ABD): hands them to the lattice
+  // field the container views, if any.
+  void commit() {
+    if (store_) store_(*this);
+  }
+  // Device address of the values of a lattice field (getRho(),
+  // getVelocity(): component-major, ghost-inclusive ASL layout, the
+  // lattice's element type) for a solver coupled on the GPU, as ASL's device
+  // containers are handed to one; nullptr for host-only containers.  stream:
+  // the consumer's hipStream_t, ordered as aslhip_macro_device orders it.
+  const void *deviceData(void *stream = nullptr) { return device_ ? device_(stream) : nullptr; }
+  // internal: storage and the fetch / store / device hooks
   std::vector<std::vector<double>> values_;
   acl::TypeID type_ = acl::TYPE_DOUBLE;
   std::function<void(AbstractData &)> fetch_;
+  std::function<void(AbstractData &)> store_;
+  std::function<const void *(void *)> device_;
 
  protected:
   Block block;

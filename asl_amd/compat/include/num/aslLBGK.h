@@ -49,6 +49,8 @@ class LBGK : public SingleKernelNM {
   // the library handle (for the BC layer and for callers of the C ABI)
   aslhip_lattice *handle() const { return h_; }
   acl::TypeID typeID() const { return type_; }
+  // the internal block the lattice was built on
+  const Block &block() const { return block_; }
   // number of state changes so far (cached host views compare against it)
   unsigned long long version() const { return version_; }
   void execute_count_bump() { ++version_; }

@@ -28,17 +28,24 @@ ABDFileOut &operator<<(ABDFileOut &f, const AbstractData &a) {
   return f;
 }
 
+// Like ASL's, these read component 0 only.  On a lattice view (getF()) the
+// other components keep their current values and the populations are written
+// back to the lattice; the rho and u views refuse the read (commit()).
 ABDFileIn &operator>>(ABDFileIn &f, AbstractData &a) {
+  a.refresh();
   std::vector<double> &v = a.values_.at(0);
   f >> std::make_pair(v.data(), static_cast<unsigned int>(v.size()));
+  a.commit();
   return f;
 }
 
 ABDFileIn &get(ABDFileIn &f, AbstractData &a, std::shared_ptr<double> d) {
+  a.refresh();
   const unsigned int s = static_cast<unsigned int>(a.values_.at(0).size());
   if (!d) d.reset(new double[s], std::default_delete<double[]>());
   f >> std::make_pair(d.get(), s);
   std::copy(d.get(), d.get() + s, a.values_[0].begin());
+  a.commit();
   return f;
 }
 

@@ -30,9 +30,46 @@ int q_of(const VectorTemplate *vT) {
   return q;
 }
 
+// Ghost-inclusive cell count of the lattice on internal block b.
+size_t lattice_len(const Block &b) { return static_cast<size_t>(productOfElements(offset(b, 1).getSize())); }
+
+// A container that is to hold (or provide) one of the lattice's fields must
+// have the lattice's ghost-inclusive size, or the library would read or write
+// past its host buffers.
+void check_len(const AbstractData &d, const Block &internal, const char *what) {
+  const size_t len = lattice_len(internal);
+  for (unsigned int c = 0; c < d.getNComponents(); ++c)
+    if (d.values_[c].size() != len)
+      errorMessage(std::string(what) + ": the container's block (ghost nodes included) is not the lattice's");
+}
+
 // A host view of a lattice field, fetched when the lattice has advanced.
-void bind_view(AbstractData &d, aslhip_lattice *h, acl::TypeID t, int field, int ncomp, const LBGK *owner) {
+// field 0: rho, 1: u, 2: f.  A reader that writes the view (ABD) commits it:
+// populations go back to the lattice; rho and u are outputs of the next
+// step and refuse.
+void bind_view(AbstractData &d, aslhip_lattice *h, acl::TypeID t, int field, int ncomp, LBGK *owner) {
   unsigned long long seen = ~0ULL;
+  d.store_ = [h, t, field, ncomp, owner](AbstractData &self) {
+    if (field != 2) errorMessage("LBGK: getRho() and getVelocity() are outputs of the method; restore getF() instead");
+    const size_t len = self.values_[0].size();
+    if (t == acl::TYPE_DOUBLE) {
+      std::vector<double> buf(len * ncomp);
+      for (int c = 0; c < ncomp; ++c) std::copy(self.values_[c].begin(), self.values_[c].end(), buf.begin() + c * len);
+      check_abi(aslhip_write_f(h, buf.data()));
+    } else {
+      std::vector<float> buf(len * ncomp);
+      for (int c = 0; c < ncomp; ++c)
+        for (size_t j = 0; j < len; ++j) buf[c * len + j] = static_cast<float>(self.values_[c][j]);
+      check_abi(aslhip_write_f(h, buf.data()));
+    }
+    owner->execute_count_bump();
+  };
+  if (field != 2)
+    d.device_ = [h, field](void *stream) -> const void * {
+      const void *r = nullptr, *u = nullptr;
+      check_abi(aslhip_macro_device(h, stream, &r, &u));
+      return field == 0 ? r : u;
+    };
   d.fetch_ = [h, t, field, ncomp, owner, seen](AbstractData &self) mutable {
     if (seen == owner->version()) return;
     const size_t len = self.values_[0].size();
@@ -65,6 +102,7 @@ LBGK::LBGK(Block b, Param nu, const VectorTemplate *vT, bool compVel, bool compR
 
 LBGK::LBGK(DataD v, Param nu, const VectorTemplate *vT) : LBGK(v->getInternalBlock(), nu, vT) {
   if (v->getNComponents() != 3) errorMessage("LBGK: the velocity container must have 3 components");
+  check_len(*v, v->getInternalBlock(), "LBGK");
   vel_ = v;
   bind_view(*vel_, h_, type_, 1, 3, this);
 }
@@ -154,6 +192,12 @@ void LBGKUtilities::initF(Param vel) { initF(acl::generateVEConstant(1.), vel); 
 
 void LBGKUtilities::initF(SPAbstractDataWithGhostNodes rho, SPAbstractDataWithGhostNodes vel) {
   if (!vel || vel->getNComponents() != 3) errorMessage("LBGKUtilities::initF: velocity field needs 3 components");
+  vel->refresh();
+  check_len(*vel, num->block(), "LBGKUtilities::initF");
+  if (rho) {
+    rho->refresh();
+    check_len(*rho, num->block(), "LBGKUtilities::initF");
+  }
   const bool dbl = num->typeID() == acl::TYPE_DOUBLE;
   const size_t len = vel->component(0).size();
   std::vector<double> ud(3 * len), rd;

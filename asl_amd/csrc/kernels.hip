@@ -693,8 +693,9 @@ template <int Q, class S>
 __global__ __launch_bounds__(BLOCK) void k_unpermute(const UnpermuteArgs a) {
   using CD = Codec<S>;
   using M = typename CD::M;
-  const int idx = blockIdx.x * BLOCK + threadIdx.x;
-  if (idx >= a.len) return;
+  const int c = blockIdx.x * BLOCK + threadIdx.x;
+  if (c >= a.count) return;
+  const int idx = a.first + c;
   M *out = static_cast<M *>(a.out);
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
@@ -707,7 +708,7 @@ __global__ __launch_bounds__(BLOCK) void k_unpermute(const UnpermuteArgs a) {
     } else {
       v = M(CD::ld(a.f + inv_dir<Q>(i) * a.qs_bytes, static_cast<unsigned>(idx) * sizeof(S)));
     }
-    out[static_cast<long long>(i) * a.len + idx] = v;
+    out[static_cast<long long>(i) * a.count + c] = v;
   }
 }
 
@@ -888,7 +889,8 @@ struct WriteFF {
 template <int Q, class S>
 struct UnpermuteF {
   static hipError_t run(const UnpermuteArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL((k_unpermute<Q, S>), dim3((a.len + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, a);
+    if (a.first < 0 || a.count < 1 || a.first + a.count > a.len) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_unpermute<Q, S>), dim3((a.count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, a);
     return hipGetLastError();
   }
 };

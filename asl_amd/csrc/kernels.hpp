@@ -132,13 +132,16 @@ struct WriteFArgs {
   void *rho, *u;
 };
 
+// The cells first .. first + count - 1 (count = len: all of them) go to
+// out[i * count + cell - first].
 struct UnpermuteArgs {
   const char *f;
   long long qs_bytes;
   int sx, sy, len;
   int per, nx, ny, nz;
   int after_odd;
-  void *out;  // Q * len macroscopic type
+  int first, count;
+  void *out;  // Q * count macroscopic type
 };
 
 struct ReduceArgs {

@@ -131,11 +131,17 @@ struct aslhip_lattice {
   double *red_partial = nullptr, *red_out = nullptr;
   StepArgs *d_args = nullptr;  // [STEP_F_COUNT]: step arguments for each value of StepArgs::flags
   std::vector<StepArgs> h_args;  // host copy of d_args (empty: not uploaded yet)
-  // hipGraph chunks of ASLHIP_GRAPH_CHUNK plain steps starting with an even /
-  // odd step (the velocity-map link fills included); rebuilt after finalize
-  // and when the step flags change
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
-  unsigned graph_fl[2] = {0u, 0u};  // the step flags each graph was captured with
+  // Captured runs of plain steps (the velocity-map link fills included):
+  // full chunks of ASLHIP_GRAPH_CHUNK steps and the shorter remainders of
+  // step counts that are not a multiple of it, each for a starting parity
+  // and the step flags it was captured with; dropped after finalize.
+  struct Graph {
+    int n;
+    bool odd;
+    unsigned fl;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
   int nvl = 0;
   // parameters
   double nu = 0.1;
@@ -149,6 +155,10 @@ struct aslhip_lattice {
   ClosureParams *d_cl = nullptr;
   float *d_feat = nullptr;  // closure features scratch (7 per interior cell)
   double dx = 1., pos0[3] = {0., 0., 0.};
+  // a stream reading rho / u through aslhip_macro_device: the next work that
+  // writes them waits for what it has queued by then
+  hipStream_t consumer = nullptr;
+  hipEvent_t ev_macro = nullptr, ev_consumer = nullptr;
   // BCs and order
   std::vector<BCDesc> bcs;
   std::vector<int> order, pending;
@@ -183,14 +193,11 @@ int check(aslhip_lattice *h) {
   return 0;
 }
 
-// Destroys the captured chunks once the stream has finished any replay of them.
+// Destroys the captured runs once the stream has finished any replay of them.
 void drop_graphs(aslhip_lattice *h) {
-  if ((h->graph[0] || h->graph[1]) && h->stream) (void)hipStreamSynchronize(h->stream);
-  for (auto &g : h->graph)
-    if (g) {
-      (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+  if (!h->graphs.empty() && h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto &g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+  h->graphs.clear();
 }
 
 void free_all(aslhip_lattice *h) {
@@ -199,7 +206,20 @@ void free_all(aslhip_lattice *h) {
                   h->val, h->vl, h->vlbc, h->red_partial, h->red_out, h->d_args, h->d_cl, h->d_feat};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  if (h->ev_macro) (void)hipEventDestroy(h->ev_macro);
+  if (h->ev_consumer) (void)hipEventDestroy(h->ev_consumer);
   if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+// Work about to be queued on the lattice's stream may overwrite rho / u (and
+// the populations): it first waits for whatever the consumer stream named in
+// aslhip_macro_device has queued up to now.
+int order_after_consumer(aslhip_lattice *h) {
+  if (!h->consumer) return 0;
+  HIPCHK(hipEventRecord(h->ev_consumer, h->consumer));
+  HIPCHK(hipStreamWaitEvent(h->stream, h->ev_consumer, 0));
+  h->consumer = nullptr;
+  return 0;
 }
 
 // Relaxation rate w = 1/(3 nu + 1/2) in the compute type
@@ -660,25 +680,49 @@ int launch_one(aslhip_lattice *h, unsigned fl, std::vector<hipEvent_t> *ev, int 
   return run_vlinks(h, was_odd);
 }
 
-// Captures ASLHIP_GRAPH_CHUNK plain steps starting at parity odd.
-int build_graph(aslhip_lattice *h, bool odd) {
-  hipGraphExec_t &g = h->graph[odd ? 1 : 0];
+// At most this many captured remainders (runs shorter than a chunk) are kept.
+constexpr size_t MAX_TAIL_GRAPHS = 6;
+
+// The captured run of n plain steps starting at parity odd with the current
+// step flags, captured now if there is none.
+int get_graph(aslhip_lattice *h, int n, bool odd, hipGraphExec_t *out) {
   const unsigned fl = base_flags(h);
-  if (g) {
-    if (h->graph_fl[odd ? 1 : 0] == fl) return 0;
-    // captured with other flags (frame, compressibility, relaxation model):
-    // its launches name the old kernel variant and argument slot; a replay of
-    // it may still be in flight
+  for (const auto &g : h->graphs)
+    if (g.n == n && g.odd == odd && g.fl == fl) {
+      *out = g.exec;
+      return 0;
+    }
+  // A run captured with other flags (frame, compressibility, relaxation
+  // model) names the old kernel variant and argument slot, and the oldest
+  // remainder makes room for a new one: destroyed once replays of them have
+  // finished.
+  auto drop = [h](size_t i) -> int {
     HIPCHK(hipStreamSynchronize(h->stream));
-    (void)hipGraphExecDestroy(g);
-    g = nullptr;
+    (void)hipGraphExecDestroy(h->graphs[i].exec);
+    h->graphs.erase(h->graphs.begin() + static_cast<long>(i));
+    return 0;
+  };
+  for (size_t i = 0; i < h->graphs.size(); ++i)
+    if (h->graphs[i].n == n && h->graphs[i].odd == odd) {
+      if (int r = drop(i)) return r;
+      break;
+    }
+  if (n != ASLHIP_GRAPH_CHUNK) {
+    size_t tails = 0, oldest = 0;
+    for (size_t i = h->graphs.size(); i-- > 0;)
+      if (h->graphs[i].n != ASLHIP_GRAPH_CHUNK) {
+        ++tails;
+        oldest = i;
+      }
+    if (tails >= MAX_TAIL_GRAPHS)
+      if (int r = drop(oldest)) return r;
   }
   const bool keep = h->next_odd;
   h->next_odd = odd;
   hipGraph_t graph = nullptr;
   HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int r = 0;
-  for (int s = 0; s < ASLHIP_GRAPH_CHUNK && !r; ++s) r = launch_one(h, fl, nullptr, s);
+  for (int s = 0; s < n && !r; ++s) r = launch_one(h, fl, nullptr, s);
   hipError_t e = hipStreamEndCapture(h->stream, &graph);
   h->next_odd = keep;
   if (r) {
@@ -686,22 +730,26 @@ int build_graph(aslhip_lattice *h, bool odd) {
     return r;
   }
   if (e != hipSuccess) return fail(ASLHIP_EHIP, "ASL ERROR: graph capture: %s", hipGetErrorString(e));
+  hipGraphExec_t g = nullptr;
   e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
-  if (e != hipSuccess) {
-    g = nullptr;
-    return fail(ASLHIP_EHIP, "ASL ERROR: graph instantiate: %s", hipGetErrorString(e));
-  }
-  h->graph_fl[odd ? 1 : 0] = fl;
+  if (e != hipSuccess) return fail(ASLHIP_EHIP, "ASL ERROR: graph instantiate: %s", hipGetErrorString(e));
+  h->graphs.push_back({n, odd, fl, g});
+  *out = g;
   return 0;
 }
 
 // graph: run whole chunks of plain steps (not the raw first one, not the
-// one writing rho/u) as hipGraph replays.
+// one writing rho/u) as hipGraph replays.  consume: the BC executions
+// recorded since the last step become the active order first (a deferred
+// step runs with the order folded in when it was requested instead: the
+// executions recorded since then belong to the step after it).
 int run_steps(aslhip_lattice *h, int nsteps, bool first_after_init, bool raw_first, bool macro_last,
-              std::vector<hipEvent_t> *ev, bool graph = false) {
+              std::vector<hipEvent_t> *ev, bool graph = false, bool consume = true) {
   if (nsteps <= 0) return 0;
-  if (int r = prepare(h, raw_first && nsteps == 1)) return r;
+  if (int r = order_after_consumer(h)) return r;
+  const bool allow_open = raw_first && nsteps == 1;
+  if (int r = consume ? prepare(h, allow_open) : (h->dirty ? finalize_impl(h, allow_open) : 0)) return r;
   if (first_after_init && !raw_first) {
     // the BCs were applied to the initial state: velocity-map links too
     int r = run_vlinks(h, false);
@@ -710,10 +758,14 @@ int run_steps(aslhip_lattice *h, int nsteps, bool first_after_init, bool raw_fir
   if (int r = upload_args(h)) return r;
   for (int s = 0; s < nsteps;) {
     const bool raw = s == 0 && raw_first;
-    if (graph && !ev && !raw && nsteps - s - (macro_last ? 1 : 0) >= ASLHIP_GRAPH_CHUNK) {
-      if (int r = build_graph(h, h->next_odd)) return r;
-      HIPCHK(hipGraphLaunch(h->graph[h->next_odd ? 1 : 0], h->stream));
-      s += ASLHIP_GRAPH_CHUNK;  // an even count: the parity is unchanged
+    const int plain = nsteps - s - (macro_last ? 1 : 0);  // plain steps left
+    if (graph && !ev && !raw && plain >= 1) {
+      const int n = plain < ASLHIP_GRAPH_CHUNK ? plain : ASLHIP_GRAPH_CHUNK;
+      hipGraphExec_t g = nullptr;
+      if (int r = get_graph(h, n, h->next_odd, &g)) return r;
+      HIPCHK(hipGraphLaunch(g, h->stream));
+      s += n;
+      if (n & 1) h->next_odd = !h->next_odd;
       continue;
     }
     unsigned fl = base_flags(h);
@@ -728,11 +780,12 @@ int run_steps(aslhip_lattice *h, int nsteps, bool first_after_init, bool raw_fir
   return 0;
 }
 
-// Runs a deferred step, writing rho and u (the host is about to look).
+// Runs a deferred step, writing rho and u (the host is about to look).  It
+// uses the BC order folded in when it was requested.
 int flush_deferred(aslhip_lattice *h) {
   if (!h->deferred) return 0;
   h->deferred = false;
-  return run_steps(h, 1, h->deferred_first, h->deferred_raw, true, nullptr);
+  return run_steps(h, 1, h->deferred_first, h->deferred_raw, true, nullptr, false, false);
 }
 
 int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEvent_t> *ev) {
@@ -740,7 +793,7 @@ int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEven
   if (!h->initialized) return fail(ASLHIP_ESTATE, "ASL ERROR: LBGK::execute before initF");
   if (h->deferred) {  // a newer step follows: no need for its rho/u
     h->deferred = false;
-    int r = run_steps(h, 1, h->deferred_first, h->deferred_raw, false, nullptr);
+    int r = run_steps(h, 1, h->deferred_first, h->deferred_raw, false, nullptr, false, false);
     if (r) return r;
   }
   // the first step after initF reads raw ghost values unless a BC ran before it
@@ -751,9 +804,12 @@ int step_impl(aslhip_lattice *h, int nsteps, unsigned flags, std::vector<hipEven
   if ((flags & ASLHIP_STEP_DEFER) && nsteps > 0) {
     int r = run_steps(h, nsteps - 1, first, raw, false, ev, graph);
     if (r) return r;
-    h->deferred = true;
     h->deferred_first = first && nsteps == 1;
     h->deferred_raw = raw && nsteps == 1;
+    // the deferred step takes the BC executions recorded up to now; later
+    // ones stay pending for the step after it
+    if (int r2 = prepare(h, h->deferred_raw)) return r2;
+    h->deferred = true;
     return 0;
   }
   return run_steps(h, nsteps, first, raw, (flags & ASLHIP_STEP_WRITE_MACRO) != 0, ev, graph);
@@ -766,6 +822,37 @@ int copy_out(aslhip_lattice *h, const void *dev, void *host, size_t bytes) {
 }
 
 }  // namespace
+
+int aslhip::read_f_through(aslhip_lattice *h, void *out, void *buf, size_t buf_bytes) {
+  if (int r = flush_deferred(h)) return r;
+  const size_t cell_bytes = (size_t)h->Q * h->melem;  // one cell's Q values
+  const size_t per = std::min(buf_bytes / cell_bytes, (size_t)h->len);
+  if (per < 1) return fail(ASLHIP_EINVAL, "ASL ERROR: read_f: staging buffer smaller than one cell");
+  UnpermuteArgs a;
+  a.f = h->f;
+  a.qs_bytes = h->qs_bytes;
+  a.sx = h->sx;
+  a.sy = h->sy;
+  a.len = h->len;
+  a.per = h->periodic ? 1 : 0;
+  a.nx = h->n[0];
+  a.ny = h->n[1];
+  a.nz = h->n[2];
+  a.after_odd = h->next_odd ? 0 : 1;
+  a.out = buf;
+  char *o = static_cast<char *>(out);
+  for (size_t first = 0; first < (size_t)h->len; first += per) {
+    const size_t count = std::min(per, (size_t)h->len - first);
+    a.first = static_cast<int>(first);
+    a.count = static_cast<int>(count);
+    HIPCHK(launch_unpermute(h->Q, h->dtype, a, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // piece i of the view: cells first .. first + count - 1 of direction i
+    HIPCHK(hipMemcpy2D(o + first * h->melem, (size_t)h->len * h->melem, buf, count * h->melem, count * h->melem,
+                       (size_t)h->Q, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
 
 extern "C" {
 
@@ -783,7 +870,12 @@ int aslhip_device_info(char *buf, size_t n, int *cu_count, double *hbm_bytes) {
   hipDeviceProp_t p;
   HIPCHK(hipGetDeviceProperties(&p, dev));
   if (buf && n) {
-    std::snprintf(buf, n, "%s (%s)", p.name, p.gcnArchName);
+    // some runtimes leave the marketing name empty: then the CU count names the part
+    char name[256] = {0};
+    if (p.name[0]) std::snprintf(name, sizeof name, "%s", p.name);
+    else if (hipDeviceGetName(name, sizeof name, dev) != hipSuccess || !name[0])
+      std::snprintf(name, sizeof name, "AMD GPU with %d CUs", p.multiProcessorCount);
+    std::snprintf(buf, n, "%s (%s)", name, p.gcnArchName);
   }
   if (cu_count) *cu_count = p.multiProcessorCount;
   if (hbm_bytes) *hbm_bytes = static_cast<double>(p.totalGlobalMem);
@@ -967,6 +1059,7 @@ int aslhip_set_compute_macro(aslhip_lattice *h, int comp_rho, int comp_vel) {
 int aslhip_init_equilibrium(aslhip_lattice *h, double rho, const double u[3]) {
   if (int r = check(h)) return r;
   if (int r = flush_deferred(h)) return r;
+  if (int r = order_after_consumer(h)) return r;
   InitArgs a;
   a.f = h->f;
   a.qs_bytes = h->qs_bytes;
@@ -988,6 +1081,7 @@ int aslhip_init_equilibrium(aslhip_lattice *h, double rho, const double u[3]) {
 int aslhip_init_equilibrium_field(aslhip_lattice *h, const void *rho, const void *u) {
   if (int r = check(h)) return r;
   if (int r = flush_deferred(h)) return r;
+  if (int r = order_after_consumer(h)) return r;
   if (!u) return fail(ASLHIP_EINVAL, "ASL ERROR: initF needs a velocity field");
   void *drho = nullptr, *du = nullptr;
   const size_t n = (size_t)h->len * h->melem;
@@ -1035,6 +1129,7 @@ int aslhip_write_f(aslhip_lattice *h, const void *f) {
   if (int r = check(h)) return r;
   if (!f) return fail(ASLHIP_EINVAL, "ASL ERROR: write_f needs populations");
   if (int r = flush_deferred(h)) return r;
+  if (int r = order_after_consumer(h)) return r;
   void *d = nullptr;
   const size_t bytes = (size_t)h->len * h->Q * h->melem;
   HIPCHK(hipMalloc(&d, bytes));
@@ -1111,6 +1206,8 @@ int aslhip_bc_execute(aslhip_lattice *h, int bc_id) {
 
 int aslhip_finalize(aslhip_lattice *h) {
   if (int r = check(h)) return r;
+  // a deferred step keeps the order it was requested with
+  if (int r = flush_deferred(h)) return r;
   if (!h->pending.empty()) {
     if (h->pending != h->order) {
       h->order = h->pending;
@@ -1145,11 +1242,24 @@ int aslhip_read_velocity(aslhip_lattice *h, void *out) {
   return copy_out(h, h->u, out, (size_t)h->len * 3 * h->melem);
 }
 
-int aslhip_macro_device(aslhip_lattice *h, const void **rho, const void **u) {
+int aslhip_macro_device(aslhip_lattice *h, void *stream, const void **rho, const void **u) {
   if (int r = check(h)) return r;
   if (!rho && !u) return fail(ASLHIP_EINVAL, "ASL ERROR: macro_device: no output address");
   if (int r = flush_deferred(h)) return r;
-  HIPCHK(hipStreamSynchronize(h->stream));  // readable from any stream until the next step
+  if (!stream) {
+    HIPCHK(hipStreamSynchronize(h->stream));  // readable from any stream until the next step
+  } else {
+    if (!h->ev_macro) HIPCHK(hipEventCreateWithFlags(&h->ev_macro, hipEventDisableTiming));
+    if (!h->ev_consumer) HIPCHK(hipEventCreateWithFlags(&h->ev_consumer, hipEventDisableTiming));
+    hipStream_t c = static_cast<hipStream_t>(stream);
+    if (h->consumer && h->consumer != c) {
+      // the earlier consumer's reads still come before the next step
+      if (int r = order_after_consumer(h)) return r;
+    }
+    HIPCHK(hipEventRecord(h->ev_macro, h->stream));
+    HIPCHK(hipStreamWaitEvent(c, h->ev_macro, 0));
+    h->consumer = c;
+  }
   if (rho) *rho = h->rho;
   if (u) *u = h->u;
   return 0;
@@ -1157,24 +1267,13 @@ int aslhip_macro_device(aslhip_lattice *h, const void **rho, const void **u) {
 
 int aslhip_read_f(aslhip_lattice *h, void *out) {
   if (int r = check(h)) return r;
+  if (!out) return fail(ASLHIP_EINVAL, "ASL ERROR: read_f needs an output buffer");
   if (int r = flush_deferred(h)) return r;
   void *d = nullptr;
   const size_t bytes = (size_t)h->len * h->Q * h->melem;
   HIPCHK(hipMalloc(&d, bytes));
-  UnpermuteArgs a;
-  a.f = h->f;
-  a.qs_bytes = h->qs_bytes;
-  a.sx = h->sx;
-  a.sy = h->sy;
-  a.len = h->len;
-  a.per = h->periodic ? 1 : 0;
-  a.nx = h->n[0];
-  a.ny = h->n[1];
-  a.nz = h->n[2];
-  a.after_odd = h->next_odd ? 0 : 1;
-  a.out = d;
-  HIPCHK(launch_unpermute(h->Q, h->dtype, a, h->stream));
-  int r = copy_out(h, d, out, bytes);
+  int r = aslhip::read_f_through(h, out, d, bytes);
+  (void)hipStreamSynchronize(h->stream);
   HIPCHK(hipFree(d));
   return r;
 }
@@ -1216,6 +1315,8 @@ int surface_impl(aslhip_lattice *h, const float *map, void *out, bool flux) {
     u.ny = h->n[1];
     u.nz = h->n[2];
     u.after_odd = h->next_odd ? 0 : 1;
+    u.first = 0;
+    u.count = h->len;
     u.out = post;
     e = launch_unpermute(h->Q, h->dtype, u, h->stream);
   }
@@ -1346,7 +1447,7 @@ int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_re
       if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_XCD_REMAP, remap)) return r;
       if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_BLOCK, blk)) return r;
       double tot = 0., ker = 0.;
-      if (int r = aslhip_time_steps(h, nsteps, 0u, &tot, &ker)) return r;
+      if (int r = aslhip_time_steps(h, nsteps, 0u, &tot, &ker, nullptr)) return r;
       if (ker < best) {
         best = ker;
         bb = blk;
@@ -1360,7 +1461,8 @@ int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_re
   return 0;
 }
 
-int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags, double *ms_total, double *ms_kernel) {
+int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags, double *ms_total, double *ms_kernel,
+                      double *ms_each) {
   if (int r = check(h)) return r;
   if (int r = flush_deferred(h)) return r;
   flags &= ~ASLHIP_STEP_DEFER;
@@ -1389,7 +1491,10 @@ int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags, double *ms_
       float k = 0.f;
       HIPCHK(hipEventElapsedTime(&k, ev[2 * s], ev[2 * s + 1]));
       ksum += k;
+      if (ms_each) ms_each[s] = k;
     }
+    if (graph && ms_each)
+      for (int s = 0; s < nsteps; ++s) ms_each[s] = 0.;
     if (ms_kernel) *ms_kernel = ksum;
   }
   for (auto &e : ev) (void)hipEventDestroy(e);
@@ -1534,6 +1639,7 @@ int batch_step_impl(aslhip_batch *b, int nsteps, unsigned flags) {
     if (h->has_closure || h->turbulence)
       return fail(ASLHIP_EINVAL, "ASL ERROR: per-cell relaxation (closure, turbulence) is not supported in batches");
     if (h->next_odd != odd) return fail(ASLHIP_ESTATE, "ASL ERROR: batch lattices are at different step parities");
+    if (int r = order_after_consumer(h)) return r;
     desc[l] = step_args(h, b->block);
     desc[n + l] = desc[l];
     if (h->comp_rho) desc[n + l].flags |= STEP_F_RHO;

@@ -8,6 +8,10 @@
 
 namespace aslhip {
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+// The host view of h's populations (what aslhip_read_f returns) written to
+// host memory out, produced in pieces of at most buf_bytes through the device
+// buffer buf (a deferred step runs first).
+int read_f_through(aslhip_lattice *h, void *out, void *buf, size_t buf_bytes);
 }
 
 #define HIPCHK(expr)                                                                                        \

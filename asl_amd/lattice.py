@@ -170,13 +170,16 @@ class Lattice:
         call("aslhip_autotune", self.h, int(nsteps), ctypes.byref(b), ctypes.byref(r))
         return b.value, r.value
 
-    def time_steps(self, nsteps, write_macro=False, graph=False):
-        """(ms of the whole call, summed ms of the step kernels; 0 with graph=True)."""
+    def time_steps(self, nsteps, write_macro=False, graph=False, each=False):
+        """(ms of the whole call, summed ms of the step kernels; 0 with graph=True)
+        [, per-launch ms as a numpy array if each]."""
         tot = ctypes.c_double(0)
         ker = ctypes.c_double(0)
+        per = np.zeros(int(nsteps)) if each else None
         flags = (STEP_WRITE_MACRO if write_macro else 0) | (STEP_GRAPH if graph else 0)
-        call("aslhip_time_steps", self.h, int(nsteps), flags, ctypes.byref(tot), ctypes.byref(ker))
-        return tot.value, ker.value
+        call("aslhip_time_steps", self.h, int(nsteps), flags, ctypes.byref(tot), ctypes.byref(ker),
+             None if per is None else per.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return (tot.value, ker.value, per) if each else (tot.value, ker.value)
 
     # host views -----------------------------------------------------------
     def rho(self):
@@ -189,11 +192,15 @@ class Lattice:
         call("aslhip_read_velocity", self.h, out.ctypes.data)
         return out.reshape((3,) + self.S)
 
-    def macro_device(self):
+    def macro_device(self, stream=None):
         """Device addresses (ints) of rho and u, holding what rho() and
-        velocity() would return; valid until the next step on this lattice."""
+        velocity() would return.  stream: the consumer's hipStream_t (an int
+        handle): its later work sees the values, and the lattice's next step
+        waits for what it queued before that step; None: the host blocks, and
+        the values hold until the next step."""
         r, u = ctypes.c_void_p(), ctypes.c_void_p()
-        call("aslhip_macro_device", self.h, ctypes.byref(r), ctypes.byref(u))
+        call("aslhip_macro_device", self.h, None if stream is None else ctypes.c_void_p(stream), ctypes.byref(r),
+             ctypes.byref(u))
         return r.value, u.value
 
     def f(self):

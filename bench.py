@@ -9,23 +9,31 @@ no-slip side walls; initial u = (0.04, 0, 0) + 1e-3 * U[-1, 1] per cell and
 component from a seeded generator, u = 0 inside the sphere.  A step is one
 fused collide+stream pass over the whole lattice with every BC applied.
 
-The step kernel runs with the library's default launch settings (512-cell
-workgroups, runs of 4 chunks per XCD; reported under roofline.launch), so
-that the rocprofv3 summaries of the same command time the same launches;
---autotune lets the library's autotuner choose them first (every workgroup
-size and XCD run length timed over 20 real steps).  The timed region holds exactly --steps steps (hipGraph replays of 100-step
-chunks) with the populations already in HBM; it is bracketed by stream
-synchronisations.  ``value`` is MLUPS over computation nodes (solid and
-ghost cells are not counted) from the wall clock around it.  The roofline
-entry is the step kernel's algorithmic HBM bytes per launch over its average
-launch duration, measured afterwards with HIP events recorded around each of
-up to 1000 eager launches on the lattice's stream.
+The step kernel runs with the library's default launch settings (reported
+under roofline.launch), so that the rocprofv3 summaries of the same command
+time the same launches; --autotune lets the library's autotuner choose them
+first (every workgroup size and XCD run length timed over 20 real steps).
+The warm-up steps are followed by untimed runs of --steps steps that capture
+the hipGraphs the timed region replays (100-step chunks and the remainder,
+for both starting parities when --steps is odd).  The timed region then holds
+exactly --steps steps, every one a graph replay, with the populations already
+in HBM; it is bracketed by stream synchronisations.  ``value`` is MLUPS over
+computation nodes (solid and ghost cells are not counted) from the wall clock
+around it.  The roofline entry is the step kernel's algorithmic HBM bytes per
+launch over its average launch duration, measured after the timed region
+with HIP events recorded on the lattice's stream around each of 400
+back-to-back eager launches (minimum and median reported too); its traffic
+figure is read from the committed rocprofv3 counter summary.
 ``cpu_baseline`` times the CPU oracle (the C++ restatement of ASL's LBGK) on
-a bounded sample of the same lattice with the host's cores.
+a bounded sample of the same lattice with every CPU the process may use (no
+more than an OMP_NUM_THREADS cap of the environment); the host's CPU facts
+are recorded with it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--autotune] [--remap G]
        python bench.py --copy-calibration BYTES   (one plain device copy, for
        scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters)
+       python bench.py --pattern-probe   (device copy and the bare D3Q19 AA
+           pattern in several layouts: the ceilings the step kernel is judged against)
        python bench.py --precision-study N [--snapshots 500,2000,5000]
            (D3Q27 periodic Taylor-Green vortex on N^3, tau = 0.5005: FP32, FP16
            and BF16 storage against FP64; one JSON line per storage type)
@@ -56,6 +64,7 @@ SIZE = (256, 128, 128)
 NU = (0.56 - 0.5) / 3.0
 U_IN = 0.04
 SEED = 20261202
+KERNEL_LAUNCHES = 400  # eager step launches timed one by one for the roofline entry
 
 
 def splitmix64_uniform(seed, n):
@@ -126,11 +135,45 @@ def build_gpu(mapv, u, storage="f32"):
     return L
 
 
+def host_cpu():
+    """What the host offers the CPU baseline: logical CPUs, the ones this
+    process may run on, the OpenMP thread cap set in the environment, and the
+    CPU model, sockets and cores per socket from lscpu."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity"] = os.cpu_count()
+    if os.environ.get("OMP_NUM_THREADS"):
+        info["omp_num_threads_env"] = os.environ["OMP_NUM_THREADS"]
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                info[k.strip()] = v.strip()
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def baseline_threads(host):
+    """All CPUs this process may use, but no more than an OpenMP cap the
+    environment sets (the share of the host granted to one GPU's job)."""
+    n = host.get("affinity") or 1
+    cap = host.get("omp_num_threads_env", "")
+    if cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return n
+
+
 def cpu_baseline(mapv, u, budget_s=15.0):
     """The CPU oracle (FP32, ASL's algorithm) on the same lattice for ~budget_s seconds."""
     import oracle
     from oracle.lbgk import set_threads
-    threads = set_threads(min(16, os.cpu_count() or 1))
+    host = host_cpu()
+    threads = set_threads(baseline_threads(host))
     O = oracle.OracleLattice(SIZE, 19, fp64=False, nu=NU)
     O.init_f(1.0, u.astype(np.float64))
     bcs = [O.add_slice_bc(O.CONSTANT_VELOCITY, 1, u=(U_IN, 0.0, 0.0)),
@@ -150,7 +193,8 @@ def cpu_baseline(mapv, u, budget_s=15.0):
     counted = int(((mapv > 0)[1:-1, 1:-1, 1:-1]).sum())
     return {"value": round(counted * steps / el / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": "CPU oracle (C++/OpenMP, FP32, ASL's pool-shift LBGK + BCs) on the same 256x128x128 "
-                      "sphere lattice, %d steps in %.1f s" % (steps, el)}
+                      "sphere lattice, %d steps in %.1f s on %d threads" % (steps, el, threads),
+            "host": host}
 
 
 def copy_calibration(nbytes):
@@ -161,6 +205,51 @@ def copy_calibration(nbytes):
     b.copy_(a)
     torch.cuda.synchronize()
     print("copied %d bytes device to device" % (a.numel() * 4), flush=True)
+
+
+def pattern_probe(reps=60):
+    """Memory-system ceilings behind the step kernel (asl_amd/libaslprobe.so).
+
+    1. A plain device copy of 1 GiB (one and four floats per lane): the
+       bandwidth a read+write stream reaches on this device.
+    2. The bare D3Q19 AA pattern on the sphere channel's 256x128x128 interior
+       (19 FP32 reads + 19 FP32 writes per cell, no boundary logic): ASL's
+       layout and rows padded to 128-byte lines, 1 / 2 / 4 cells per lane,
+       256- and 512-lane workgroups.  Algorithmic TB/s = 152 B x cells over
+       the mean of the even and odd launch medians.
+    """
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "asl_amd", "libaslprobe.so"))
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.aslprobe_copy.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_int, dp]
+    lib.aslprobe_aa.argtypes = [ctypes.c_int] * 8 + [dp]
+    nbytes = 1 << 30
+    for vec in (1, 4):
+        ms = np.zeros(reps)
+        rc = lib.aslprobe_copy(nbytes, vec, reps, ms.ctypes.data_as(dp))
+        if rc:
+            sys.exit("pattern probe: copy failed (%d)" % rc)
+        med = float(np.median(ms[2:]))
+        print(json.dumps({"probe": "device copy", "bytes_copied": nbytes, "floats_per_lane": vec,
+                          "ms_median": round(med, 5), "tbps_read_plus_write": round(2 * nbytes / med / 1e9, 3),
+                          "frac_of_8tbps": round(2 * nbytes / med / 1e9 / 8.0, 4)}), flush=True)
+    nx, ny, nz = SIZE
+    cells = nx * ny * nz
+    for layout, lname in ((0, "ASL rows (nz + 2)"), (1, "rows padded to 128-byte lines")):
+        for cpl in (1, 2, 4):
+            for block in (256, 512):
+                for remap in (4, 16):
+                    ms = np.zeros(reps)
+                    rc = lib.aslprobe_aa(nx, ny, nz, layout, cpl, block, remap, reps, ms.ctypes.data_as(dp))
+                    if rc:
+                        sys.exit("pattern probe: AA failed (%d)" % rc)
+                    ev, od = float(np.median(ms[2::2])), float(np.median(ms[3::2]))
+                    tb = 152.0 * cells / ((ev + od) / 2) / 1e9
+                    print(json.dumps({"probe": "D3Q19 AA pattern, 256x128x128 interior", "layout": lname,
+                                      "cells_per_lane": cpl, "workgroup": block, "xcd_run": remap,
+                                      "ms_even": round(ev, 5), "ms_odd": round(od, 5),
+                                      "tbps_algorithmic": round(tb, 3), "frac_of_8tbps": round(tb / 8.0, 4)}),
+                          flush=True)
 
 
 def precision_study(n, snapshots, u0=0.04, tau=0.5005):
@@ -421,13 +510,20 @@ def variant_study(n, steps=500):
         del L
 
 
+TRAFFIC_PROFILE = os.path.join("profiles", "lbgk_step_d3q19_f32_sphere_256x128x128_traffic.md")
+
+
 def profile_traffic():
-    """HBM bytes per launch of the step kernel from the committed rocprofv3 summary, or None."""
-    path = os.path.join(ROOT, "profiles", "lbgk_step_d3q19_f32_sphere_256x128x128_traffic.md")
+    """(HBM bytes per launch of the step kernel, where the figure comes from):
+    read from the committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated); (None, None) if it is absent."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
     if not os.path.exists(path):
-        return None
+        return None, None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
-    return float(m.group(1)) if m else None
+    if not m:
+        return None, None
+    return float(m.group(1)), TRAFFIC_PROFILE + " (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
 
 
 def main():
@@ -462,7 +558,12 @@ def main():
                     help="batch study: lattice-group size instead of the group autotuner's choice")
     ap.add_argument("--batch-study", action="store_true",
                     help="only run the batched-lattices study (64 cavities of uneven sizes)")
+    ap.add_argument("--pattern-probe", action="store_true",
+                    help="only run the memory-system probes: device copy and the bare AA access pattern")
     args = ap.parse_args()
+    if args.pattern_probe:
+        pattern_probe()
+        return
     if args.cavity_study:
         cavity_study(args.no_cpu)
         return
@@ -506,23 +607,28 @@ def main():
     interior = int(np.prod(SIZE))
     if args.warmup:
         L.step(args.warmup, write_macro=False, graph=True)
+    # Capture the graphs the timed steps replay (a chunk of 100 and the
+    # remainder), for both starting parities when the step count is odd, so
+    # that the timed region only replays them.
+    for _ in range(2 if args.steps % 2 else 1):
+        L.step(args.steps, write_macro=False, graph=True)
     L.synchronize()
-    # the timed steps: hipGraph-replayed chunks of 100 launches
+    # the timed steps: every one a hipGraph replay
     t0 = time.perf_counter()
     ms_graph, _ = L.time_steps(args.steps, write_macro=False, graph=True)
     L.synchronize()
     wall = time.perf_counter() - t0
     mlups = counted * args.steps / wall / 1e6
-    # the step kernel's own duration: HIP events around each launch
-    nk = min(args.steps, 1000)
-    _, ms_kernel = L.time_steps(nk, write_macro=False)
+    # the step kernel's own duration: HIP events on the lattice's stream
+    # around each of KERNEL_LAUNCHES back-to-back eager launches
+    _, _, each = L.time_steps(KERNEL_LAUNCHES, write_macro=False, each=True)
 
     # algorithmic bytes per launch: 19 FP32 reads + 19 FP32 writes per
     # computation node, plus the one-byte cell type of every interior cell
     bytes_per_launch = 152.0 * counted + 1.0 * interior
-    kern_s = ms_kernel / nk / 1e3
+    kern_s = float(np.mean(each)) / 1e3
     achieved = bytes_per_launch / kern_s / 1e9
-    traffic = profile_traffic()
+    traffic, traffic_src = profile_traffic()
     line = {
         "metric": METRIC,
         "value": round(mlups, 1),
@@ -539,10 +645,12 @@ def main():
         "config": {"workload": "D3Q19 LBGK channel flow past a sphere, 256x128x128, FP32, tau=0.56, "
                                "sphere r=16 at (64,64,64), inlet u=0.04 (X0), outlet rho=1 (XE), "
                                "no-slip walls", "lattice": "D3Q19", "cells": interior,
-                   "computation_nodes": counted, "storage": "FP32 SoA, in-place AA pattern"},
+                   "computation_nodes": counted, "storage": "FP32 SoA, in-place AA pattern",
+                   "timed_steps_graph_replayed": args.steps, "timed_steps_eager": 0},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel_ms_avg": round(kern_s * 1e3, 5), "kernel_launches_timed": nk,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel_ms_avg": round(kern_s * 1e3, 5), "kernel_ms_min": round(float(np.min(each)), 5),
+                     "kernel_ms_median": round(float(np.median(each)), 5), "kernel_launches_timed": KERNEL_LAUNCHES,
                      "bytes_per_launch": bytes_per_launch, "graph_ms_per_step": round(ms_graph / args.steps, 5),
                      "launch": {"workgroup": block, "xcd_run": remap}},
         "cpu_baseline": None,

@@ -80,8 +80,12 @@ extern "C" {
 #define ASLHIP_STEP_DEFER 2u       /* keep the last step pending: it runs, writing
                                       rho and u, when the host next reads or
                                       synchronises, or before the next step */
-#define ASLHIP_STEP_GRAPH 4u       /* run whole chunks of ASLHIP_GRAPH_CHUNK
-                                      steps as replays of captured hipGraphs */
+#define ASLHIP_STEP_GRAPH 4u       /* run the plain steps (all but a first step
+                                      that reads raw ghost values and a last
+                                      one that writes rho and u) as replays of
+                                      captured hipGraphs: chunks of
+                                      ASLHIP_GRAPH_CHUNK steps, then one run of
+                                      the remainder (captured on first use) */
 #define ASLHIP_GRAPH_CHUNK 100
 
 typedef struct aslhip_lattice aslhip_lattice;
@@ -267,12 +271,19 @@ int aslhip_read_velocity(aslhip_lattice *h, void *out);
 /* Device addresses of rho (one array) and u (three arrays, component-major),
  * in the ghost-inclusive ASL layout and the element type aslhip_read_rho /
  * aslhip_read_velocity copy (double for ASLHIP_F64 lattices, float
- * otherwise), holding what those reads would return: a deferred step runs
- * first and the lattice's stream is synchronised, so any stream may read
- * them until the next step on this lattice.  For a solver coupled on the GPU
- * to lbgk->getVelocity() (examples/flow/multiphase_flow.cc:182-184, which
- * hands it to asl::generateFDMultiPhase).  rho or u may be NULL. */
-int aslhip_macro_device(aslhip_lattice *h, const void **rho, const void **u);
+ * otherwise), holding what those reads would return (a deferred step runs
+ * first).  For a solver coupled on the GPU to lbgk->getVelocity()
+ * (examples/flow/multiphase_flow.cc:182-184, which hands it to
+ * asl::generateFDMultiPhase).  rho or u may be NULL.
+ * Ordering: stream is the consumer's hipStream_t.  Work queued on it after
+ * this call sees the values (it waits for the lattice's stream, the host does
+ * not block).  The next call that overwrites them on this lattice (a step,
+ * initF, write_f, a batch step) first waits for everything queued on that
+ * stream up to the moment of that call; work the consumer queues later may
+ * see the newer values.  stream NULL: the host synchronises with the
+ * lattice's stream instead, and the caller must not read them after the next
+ * step is requested. */
+int aslhip_macro_device(aslhip_lattice *h, void *stream, const void **rho, const void **u);
 int aslhip_read_f(aslhip_lattice *h, void *out);
 
 /* Packed cell-type map (one byte per ghost-inclusive cell):
@@ -317,9 +328,11 @@ int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_re
 /* Times nsteps steps with HIP events recorded on the lattice's stream.
  * ms_total: wall time of the whole call between the events; ms_kernel:
  * summed time of the step kernels (events around each launch; 0 with
- * ASLHIP_STEP_GRAPH, whose replays carry no events between launches). */
+ * ASLHIP_STEP_GRAPH, whose replays carry no events between launches);
+ * ms_each: nsteps doubles, the duration of each launch (0 with
+ * ASLHIP_STEP_GRAPH).  Any output may be NULL. */
 int aslhip_time_steps(aslhip_lattice *h, int nsteps, unsigned flags,
-                      double *ms_total, double *ms_kernel);
+                      double *ms_total, double *ms_kernel, double *ms_each);
 
 /* ---- learned closure ------------------------------------------------
  * Extension (no counterpart in ASL): a per-cell relaxation-time correction

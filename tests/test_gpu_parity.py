@@ -69,6 +69,31 @@ def test_first_step_reads_raw_ghosts_when_no_bc_was_executed():
     assert d["rho"] == 0.0 and d["u"] == 0.0, d
 
 
+@pytest.mark.parametrize("deferred_steps", [1, 3])
+def test_deferred_step_keeps_its_bc_order(deferred_steps):
+    """asl::LBGK::execute() leaves its step pending; BCs executed after it in
+    a changed order belong to the step after it.  Driver sequence: BCs (lid,
+    walls); execute(); BCs (walls, lid); 30 more steps.  The pending step must
+    still let the walls win the lid edges, as ASL's own order does."""
+    G, O = pair((12, 11, 10), 19, nu=(0.6 - 0.5) / 3)
+    G.init_equilibrium(1.0, (0, 0, 0))
+    O.init_f(1.0, (0, 0, 0))
+    lid = add_bc(G, O, _lib.BC_CONSTANT_VELOCITY, _lib.ZE, u=(0.05, 0, 0))
+    walls = add_bc(G, O, _lib.BC_NOSLIP, ALL_BUT_ZE)
+    for g, o in (lid, walls):
+        G.execute_bc(g)
+        O.execute_bc(o)
+    G.step(deferred_steps, defer=True)
+    G.execute_bc(walls[0])
+    G.execute_bc(lid[0])
+    G.step(30)
+    # the steps of one call repeat the active order between them
+    O.run(deferred_steps - 1, [lid[1], walls[1]])
+    O.run(1 + 30, [walls[1], lid[1]])
+    d = compare(G, O)
+    assert d["rho"] == 0.0 and d["u"] == 0.0 and d["f"] == 0.0, d
+
+
 def test_pure_streaming_bitwise():
     # nu = inf: w = 0, populations only stream and bounce
     n = 12
