@@ -26,6 +26,7 @@ TUNE_WIDE = 4
 DF_SPHERE, DF_CYLINDER, DF_CONE, DF_PLANE, DF_AND, DF_OR, DF_NEG, DF_NORMALIZE = 1, 2, 3, 4, 5, 6, 7, 8
 CLOSURE_NPARAM = 1345
 CLOSURE_FP32, CLOSURE_BF16_MFMA = 0, 1
+VIS_NOSLIP_VEL, VIS_NOSLIP_RHO, VIS_VELOCITY = 1, 2, 3
 
 # Every entry point of include/aslhip.h with its ctypes signature.
 _vp = ctypes.c_void_p
@@ -59,6 +60,8 @@ SIGNATURES = {
     "aslhip_write_f": [_vp, _vp],
     "aslhip_surface_force": [_vp, _fp, _vp],
     "aslhip_surface_flux": [_vp, _fp, _vp],
+    "aslhip_vis_add": [_vp, ctypes.c_int, _fp, _dp, _dp, _ip],
+    "aslhip_vis_execute": [_vp, ctypes.c_int],
     "aslhip_template": [ctypes.c_int, _ip, _dp, _ip],
     "aslhip_batch_autotune_groups": [_vp, ctypes.c_int, _ip],
     "aslhip_get_tuning": [_vp, _ip, _ip],

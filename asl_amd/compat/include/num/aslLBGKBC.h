@@ -86,37 +86,25 @@ class BCConstantPressureVelocityMap : public BCondWithMap {
                                 SPAbstractDataWithGhostNodes map);
 };
 
-/// Visualisation fill of a macroscopic field at map ghost nodes, on the host
-/// view (ASL's BCConstantValueMiddlePointMap, src/num/aslBasicBC.cxx:171-230):
-/// v(g) = sum_i lc_i (2 value - v(g + c_i)) / max(sum_i lc_i, 0.01) over the
-/// computation-node neighbours; with gradient = true the zero-gradient fill
-/// v(g) = sum_i lc_i v(g + c_i) / sum_i lc_i.
+/// Visualisation fills of an LBGK's rho or velocity at the boundary nodes of
+/// a map (map <= 0 and > -0.9999), on the GPU in place (aslhip_vis_add /
+/// aslhip_vis_execute), behind ASL's generateBCNoSlipVel, generateBCNoSlipRho
+/// and generateBCVelocityVel (src/num/aslLBGKBC.cxx:808-852): the kernels of
+/// BCConstantValueMiddlePointMap, BCConstantGradientMap and BCValuePFMap
+/// (src/num/aslBasicBC.cxx:124-145, 167-227, 283-351).  init() copies the
+/// map to the device; execute() fills and marks the host views stale.
 class BCMacroFillMap : public NumMethod {
  public:
-  BCMacroFillMap(SPLBGK nm, SPAbstractDataWithGhostNodes data, SPAbstractDataWithGhostNodes map, bool gradient);
-  virtual void init() {}
+  BCMacroFillMap(SPLBGK nm, int kind, SPAbstractDataWithGhostNodes map, SPPositionFunction v = nullptr);
+  virtual void init();
   virtual void execute();
 
  private:
   SPLBGK num;
-  SPAbstractDataWithGhostNodes data, map;
-  bool gradient;
-};
-
-/// Visualisation fill of a field with a position function at the map's BC
-/// nodes (map <= 0 and > -0.9999), on the host view: ASL's BCValuePFMap
-/// (src/num/aslBasicBC.cxx:114-148), behind generateBCVelocityVel
-/// (src/num/aslLBGKBC.cxx:847-852).
-class BCValuePFMap : public NumMethod {
- public:
-  BCValuePFMap(SPAbstractDataWithGhostNodes data, SPPositionFunction v, SPAbstractDataWithGhostNodes map);
-  virtual void init() {}
-  virtual void execute();
-
- private:
-  SPAbstractDataWithGhostNodes data;
-  SPPositionFunction value;
+  int kind_;
   SPAbstractDataWithGhostNodes map;
+  SPPositionFunction value;
+  int id_ = -1;
 };
 
 // ComputeSurfaceFluxMap / ComputeSurfaceForceMap (src/num/aslLBGKBC.h:247-279):

@@ -128,73 +128,35 @@ BCConstantPressureVelocityMap::BCConstantPressureVelocityMap(SPLBGK nm, acl::Vec
   vec3(v, u_, "BCConstantPressureVelocityMap");
 }
 
-BCMacroFillMap::BCMacroFillMap(SPLBGK nm, SPAbstractDataWithGhostNodes d, SPAbstractDataWithGhostNodes m,
-                               bool grad)
-    : num(nm), data(d), map(m), gradient(grad) {}
-
-void BCMacroFillMap::execute() {
-  const std::vector<float> m = map_array(*num, map, "BCMacroFillMap");
-  const VectorTemplate &t = *num->getVectorTemplate();
-  const Block &b = data->getBlock();
-  const long len = static_cast<long>(m.size());
-  const unsigned int nc = data->getNComponents();
-  std::vector<std::vector<double>> src(nc);
-  for (unsigned int c = 0; c < nc; ++c) src[c] = data->component(c);
-  std::vector<int> off(t.vectors.size());
-  for (size_t i = 0; i < t.vectors.size(); ++i) off[i] = b.c2i(t.vectors[i]);
-  auto mapx = [&](long i) { return (i < 0 || i > len - 1) ? -.9f : m[i]; };
-  for (long g = 0; g < len; ++g) {
-    if (!(m[g] <= 0.f && m[g] > -.9999f)) continue;
-    double counter = 0., acc[3] = {0., 0., 0.};
-    for (size_t i = 1; i < t.vectors.size(); ++i) {
-      const long n = g + off[i];
-      if (!(mapx(n) > 0.f)) continue;
-      counter += t.laplasCoefs[i];
-      for (unsigned int c = 0; c < nc; ++c) {
-        const double vb = src[c][(len + n) % len];
-        acc[c] += (gradient ? vb : -vb) * t.laplasCoefs[i];  // value 0: 2*0 - v
-      }
-    }
-    for (unsigned int c = 0; c < nc; ++c) data->values_[c][g] = acc[c] / std::max(counter, .01);
-  }
+BCMacroFillMap::BCMacroFillMap(SPLBGK nm, int kind, SPAbstractDataWithGhostNodes m, SPPositionFunction v)
+    : num(nm), kind_(kind), map(m), value(v) {
+  if (kind == ASLHIP_VIS_VELOCITY && !value) errorMessage("BCValuePFMap: no position function");
 }
 
-BCValuePFMap::BCValuePFMap(SPAbstractDataWithGhostNodes d, SPPositionFunction v, SPAbstractDataWithGhostNodes m)
-    : data(d), value(v), map(m) {}
-
-// value(position) at the guarded map nodes; position = block position + dx * index
-// (Index2PositionACL, src/math/aslIndex2Position.cxx:64-108).  PFConstant or
-// PFRotationField axis x (position - centre) (src/math/aslPositionFunction.cxx:187-197).
-void BCValuePFMap::execute() {
-  if (!value) errorMessage("BCValuePFMap: no position function");
-  const Block &b = data->getBlock();
-  if (map->getBlock().getSize() != b.getSize()) errorMessage("BCValuePFMap: the map's block differs from the field's");
-  const unsigned int nc = data->getNComponents();
-  if (nc != 3) errorMessage("BCValuePFMap: a 3-component field is expected");
-  const std::vector<double> &m = map->component(0);
-  std::vector<std::vector<double>> v(nc);
-  for (unsigned int c = 0; c < nc; ++c) v[c] = data->component(c);
-  const int s0 = b.getSize()[0], s1 = b.getSize()[1], s2 = b.getSize()[2];
-  for (int i = 0; i < s0; ++i)
-    for (int j = 0; j < s1; ++j)
-      for (int k = 0; k < s2; ++k) {
-        const long g = (static_cast<long>(i) * s1 + j) * s2 + k;
-        const float mg = static_cast<float>(m[g]);
-        if (!(mg <= 0.f && mg > -.9999f)) continue;
-        const double p[3] = {b.position[0] + b.dx * i, b.position[1] + b.dx * j, b.position[2] + b.dx * k};
-        double u[3];
-        if (value->rotation) {
-          const double r[3] = {p[0] - value->centre[0], p[1] - value->centre[1], p[2] - value->centre[2]};
-          const AVec<double> &a = value->axis;
-          u[0] = a[1] * r[2] - a[2] * r[1];
-          u[1] = a[2] * r[0] - a[0] * r[2];
-          u[2] = a[0] * r[1] - a[1] * r[0];
-        } else {
-          for (int d = 0; d < 3; ++d) u[d] = value->constant[d];
-        }
-        for (unsigned int c = 0; c < nc; ++c) v[c][g] = u[c];
+void BCMacroFillMap::init() {
+  if (id_ >= 0) return;
+  const std::vector<float> m = map_array(*num, map, "BCMacroFillMap");
+  double u[3] = {0., 0., 0.}, rot[6] = {0., 0., 0., 0., 0., 0.};
+  bool has_rot = false;
+  if (kind_ == ASLHIP_VIS_VELOCITY) {
+    if (value->rotation) {
+      has_rot = true;
+      for (int d = 0; d < 3; ++d) {
+        rot[d] = value->axis[d];
+        rot[3 + d] = value->centre[d];
       }
-  for (unsigned int c = 0; c < nc; ++c) data->values_[c] = v[c];
+    } else {
+      if (value->constant.getSize() != 3) errorMessage("BCValuePFMap: velocity must have 3 components");
+      for (int d = 0; d < 3; ++d) u[d] = value->constant[d];
+    }
+  }
+  check_abi(aslhip_vis_add(num->handle(), kind_, m.data(), u, has_rot ? rot : nullptr, &id_));
+}
+
+void BCMacroFillMap::execute() {
+  if (id_ < 0) init();
+  check_abi(aslhip_vis_execute(num->handle(), id_));
+  num->execute_count_bump();  // the host views of rho and u fetch the filled values
 }
 
 ComputeSurfaceFluxMap::ComputeSurfaceFluxMap(SPLBGK nm, SPDataWithGhostNodesACLData fF,
@@ -279,15 +241,15 @@ SPNumMethod generateBCNoSlip(SPLBGK nm, SPAbstractDataWithGhostNodes map) {
 }
 
 SPNumMethod generateBCNoSlipVel(SPLBGK nmU, SPAbstractDataWithGhostNodes map) {
-  return std::make_shared<BCMacroFillMap>(nmU, nmU->getVelocity(), map, false);
+  return std::make_shared<BCMacroFillMap>(nmU, ASLHIP_VIS_NOSLIP_VEL, map);
 }
 
 SPNumMethod generateBCNoSlipRho(SPLBGK nmU, SPAbstractDataWithGhostNodes map) {
-  return std::make_shared<BCMacroFillMap>(nmU, nmU->getRho(), map, true);
+  return std::make_shared<BCMacroFillMap>(nmU, ASLHIP_VIS_NOSLIP_RHO, map);
 }
 
 SPNumMethod generateBCVelocityVel(SPLBGK nm, SPPositionFunction v, SPAbstractDataWithGhostNodes map) {
-  return std::make_shared<BCValuePFMap>(nm->getVelocity(), v, map);
+  return std::make_shared<BCMacroFillMap>(nm, ASLHIP_VIS_VELOCITY, map, v);
 }
 
 SPNumMethod generateBCVelocity(SPLBGK nm, SPPositionFunction v, SPAbstractDataWithGhostNodes map) {

@@ -524,6 +524,125 @@ __global__ __launch_bounds__(BLOCK) void k_surface(const SurfaceArgs a) {
   }
 }
 
+// Visualisation fills of rho / u (VisArgs).  Reads come only from cells with
+// map > 0 and writes go only to cells with map <= 0, so the update in place
+// is free of races.  The operations follow the CPU oracle's restatement of
+// ASL's kernels (src/num/aslBasicBC.cxx:124-145, 167-227, 283-351) in its
+// order, none contracted (the library is built with -ffp-contract=off):
+//   NOSLIP_VEL: v += select(0 - v(x + c_i), b_i) * lc_i, n += select(lc_i, b_i),
+//               v(x) = v / max(n, .01), over i = 1..Q-1, b_i = map(x + c_i) > 0;
+//   NOSLIP_RHO: g_k = sum_i map(x + c_i) (gc_i c_i[k]) over c_i[k] != 0,
+//               g /= |g|, p_i = g . c_i/|c_i|, d_i = (p_i + 1)^2,
+//               n = max(sum_i select(d_i, b_i), .001),
+//               r = sum_i select((rho(x + c_i) - p_i * 0) d_i, b_i),
+//               rho(x) = r / n where n > 0.1;
+//   VELOCITY:   u(x) = u0, or axis x (position - centre).
+template <int Q, class S>
+__global__ __launch_bounds__(BLOCK) void k_vis(const VisArgs a) {
+  using M = typename Codec<S>::M;
+  const int g = blockIdx.x * BLOCK + threadIdx.x;
+  if (g >= a.len) return;
+  const float mg = a.map[g];
+  if (!(mg <= 0.f && mg > -.9999f)) return;
+  M *rho = static_cast<M *>(a.rho), *u = static_cast<M *>(a.u);
+  const long long len = a.len;
+  auto mx = [&](long long y) { return (y < 0 || y > len - 1) ? -.9f : a.map[y]; };
+  auto wrap = [&](long long y) { return (len + y) % len; };
+  if (a.kind == ASLHIP_VIS_VELOCITY) {
+    M uw[3] = {M(a.u0[0]), M(a.u0[1]), M(a.u0[2])};
+    if (a.has_rot) {
+      // Index2PositionACL over the ghost-inclusive block (src/math/aslIndex2Position.cxx:64-108)
+      const int I0 = g / a.sx, rem = g - I0 * a.sx, I1 = rem / a.sy, I2 = rem - I1 * a.sy;
+      const int I[3] = {I0, I1, I2};
+      M pos[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) pos[d] = M(a.pos0[d] - a.dx) + M(a.dx) * M(I[d]);
+      const M d0 = pos[0] - M(a.rot[3]), d1 = pos[1] - M(a.rot[4]), d2 = pos[2] - M(a.rot[5]);
+      uw[0] = M(a.rot[1]) * d2 - M(a.rot[2]) * d1;
+      uw[1] = M(a.rot[2]) * d0 - M(a.rot[0]) * d2;
+      uw[2] = M(a.rot[0]) * d1 - M(a.rot[1]) * d0;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[c * len + g] = uw[c];
+    return;
+  }
+  if constexpr (Q != 27) {
+    using LT = Lat<Q>;
+    if (a.kind == ASLHIP_VIS_NOSLIP_VEL) {
+      M v[3] = {M(0), M(0), M(0)}, n = M(0);
+#pragma unroll
+      for (int i = 1; i < Q; ++i) {
+        const long long y = g + offc<Q>(i, a.sx, a.sy);
+        const bool b = mx(y) > 0.f;
+        const M l = M(LT::lc(i));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = v[c] + (b ? M(0) - u[c * len + wrap(y)] : M(0)) * l;
+        n = n + (b ? l : M(0));
+      }
+      const M den = n > M(.01) ? n : M(.01);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) u[c * len + g] = v[c] / den;
+    } else if (a.kind == ASLHIP_VIS_NOSLIP_RHO) {
+      M mv[Q];
+      bool bv[Q];
+#pragma unroll
+      for (int i = 1; i < Q; ++i) {
+        const float m = mx(g + offc<Q>(i, a.sx, a.sy));
+        mv[i] = M(m);
+        bv[i] = m > 0.f;
+      }
+      M gn[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        M acc = M(0);
+        bool first = true;
+#pragma unroll
+        for (int i = 1; i < Q; ++i) {
+          if (LT::c[i][k] == 0) continue;
+          const M t = mv[i] * M(LT::gc(i) * double(LT::c[i][k]));
+          acc = first ? t : acc + t;
+          first = false;
+        }
+        gn[k] = acc;
+      }
+      M l2 = gn[0] * gn[0];
+      l2 = l2 + gn[1] * gn[1];
+      l2 = l2 + gn[2] * gn[2];
+      const M ln = sizeof(M) == 4 ? M(__builtin_sqrtf(float(l2))) : M(__builtin_sqrt(double(l2)));
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gn[k] = gn[k] / ln;
+      M pr[Q], dc[Q];
+      M n = M(0);
+#pragma unroll
+      for (int i = 1; i < Q; ++i) {
+        const double cl = __builtin_sqrt(double(LT::c[i][0] * LT::c[i][0] + LT::c[i][1] * LT::c[i][1] +
+                                                LT::c[i][2] * LT::c[i][2]));
+        M p = M(0);
+        bool first = true;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if (LT::c[i][d] == 0) continue;
+          const M t = gn[d] * M(double(LT::c[i][d]) / cl);
+          p = first ? t : p + t;
+          first = false;
+        }
+        pr[i] = p;
+        const M e = p + M(1.);
+        dc[i] = e * e;
+        n = n + (bv[i] ? dc[i] : M(0));
+      }
+      n = n > M(.001) ? n : M(.001);
+      M r = M(0);
+#pragma unroll
+      for (int i = 1; i < Q; ++i) {
+        const long long y = g + offc<Q>(i, a.sx, a.sy);
+        r = r + (bv[i] ? (rho[wrap(y)] - pr[i] * M(0)) * dc[i] : M(0));
+      }
+      if (n > M(.1)) rho[g] = r / n;
+    }
+  }
+}
+
 // Many lattices in one launch: workgroup b works on chunk ent[b].y of
 // lattice ent[b].x, whose descriptor lives in device memory.  One chunk per
 // workgroup, with no loop around the body: a loop over several chunks kept
@@ -880,6 +999,14 @@ struct SurfaceF {
   }
 };
 template <int Q, class S>
+struct VisF {
+  static hipError_t run(const VisArgs &a, hipStream_t s) {
+    if (Q == 27 && a.kind != ASLHIP_VIS_VELOCITY) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_vis<Q, S>), dim3((a.len + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+  }
+};
+template <int Q, class S>
 struct WriteFF {
   static hipError_t run(const WriteFArgs &a, hipStream_t s) {
     hipLaunchKernelGGL((k_write_f<Q, S>), dim3((a.len + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, a);
@@ -928,6 +1055,7 @@ hipError_t launch_peek(int q, int dtype, bool odd, bool periodic, const StepLaun
 hipError_t launch_surface(int q, int dtype, const SurfaceArgs &a, hipStream_t s) {
   return dispatch<SurfaceF>(q, dtype, a, s);
 }
+hipError_t launch_vis(int q, int dtype, const VisArgs &a, hipStream_t s) { return dispatch<VisF>(q, dtype, a, s); }
 hipError_t launch_write_f(int q, int dtype, const WriteFArgs &a, hipStream_t s) {
   return dispatch<WriteFF>(q, dtype, a, s);
 }

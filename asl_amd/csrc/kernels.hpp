@@ -121,6 +121,20 @@ struct SurfaceArgs {
   int flux;               // 1: ComputeSurfaceFluxMap, 0: ComputeSurfaceForceMap
 };
 
+// Visualisation fill of a lattice's rho or u in place at the guarded boundary
+// nodes of a map (map <= 0 and > -0.9999): ASLHIP_VIS_NOSLIP_VEL,
+// ASLHIP_VIS_NOSLIP_RHO or ASLHIP_VIS_VELOCITY (include/aslhip.h).
+struct VisArgs {
+  const float *map;    // len values, ASL layout
+  void *rho, *u;       // macroscopic type
+  int kind;
+  int len, sx, sy;
+  double u0[3];        // ASLHIP_VIS_VELOCITY: the velocity (without rotation)
+  double rot[6];       // ASLHIP_VIS_VELOCITY with has_rot: axis, centre
+  int has_rot;
+  double dx, pos0[3];  // position of ghost-inclusive index 0 is pos0 - dx
+};
+
 // Populations from a host view (Q x len, f_i of each cell) into the layout a
 // step of odd parity reads (slot (x, inv(i))), with rho and u of each cell.
 struct WriteFArgs {
@@ -177,6 +191,7 @@ hipError_t launch_write_f(int q, int dtype, const WriteFArgs &a, hipStream_t s);
 // macroscopic type, other cells untouched); lattices under 4 GiB.
 hipError_t launch_peek(int q, int dtype, bool odd, bool periodic, const StepLaunch &a, void *out, hipStream_t s);
 hipError_t launch_surface(int q, int dtype, const SurfaceArgs &a, hipStream_t s);
+hipError_t launch_vis(int q, int dtype, const VisArgs &a, hipStream_t s);
 hipError_t launch_unpermute(int q, int dtype, const UnpermuteArgs &a, hipStream_t s);
 // partial must hold 5 * reduce_blocks(len) doubles; out: 5 doubles (device).
 int reduce_blocks(int len);

@@ -159,6 +159,15 @@ struct aslhip_lattice {
   // writes them waits for what it has queued by then
   hipStream_t consumer = nullptr;
   hipEvent_t ev_macro = nullptr, ev_consumer = nullptr;
+  // visualisation fills (aslhip_vis_add): kind, device copy of the map, velocity
+  struct Vis {
+    int kind;
+    float *map;
+    double u[3];
+    bool has_rot;
+    double rot[6];
+  };
+  std::vector<Vis> vis;
   // BCs and order
   std::vector<BCDesc> bcs;
   std::vector<int> order, pending;
@@ -206,6 +215,8 @@ void free_all(aslhip_lattice *h) {
                   h->val, h->vl, h->vlbc, h->red_partial, h->red_out, h->d_args, h->d_cl, h->d_feat};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  for (auto &v : h->vis)
+    if (v.map) (void)hipFree(v.map);
   if (h->ev_macro) (void)hipEventDestroy(h->ev_macro);
   if (h->ev_consumer) (void)hipEventDestroy(h->ev_consumer);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1190,6 +1201,55 @@ int aslhip_bc_add_map(aslhip_lattice *h, int kind, const float *map, const float
   h->bcs.push_back(std::move(b));
   if (bc_id) *bc_id = static_cast<int>(h->bcs.size()) - 1;
   h->dirty = true;
+  return 0;
+}
+
+int aslhip_vis_add(aslhip_lattice *h, int kind, const float *map, const double u[3], const double rot[6],
+                   int *vis_id) {
+  if (int r = check(h)) return r;
+  if (kind < ASLHIP_VIS_NOSLIP_VEL || kind > ASLHIP_VIS_VELOCITY)
+    return fail(ASLHIP_EINVAL, "ASL ERROR: %d is not a visualisation fill kind", kind);
+  if (!map) return fail(ASLHIP_EINVAL, "ASL ERROR: visualisation fill without a map");
+  if (h->Q == 27 && kind != ASLHIP_VIS_VELOCITY)
+    return fail(ASLHIP_EINVAL, "ASL ERROR: d3q27 has no laplace / gradient coefficients for this fill");
+  aslhip_lattice::Vis v{kind, nullptr, {0., 0., 0.}, rot != nullptr, {0., 0., 0., 0., 0., 0.}};
+  for (int d = 0; d < 3; ++d) v.u[d] = u ? u[d] : 0.;
+  if (rot)
+    for (int d = 0; d < 6; ++d) v.rot[d] = rot[d];
+  const size_t bytes = (size_t)h->len * sizeof(float);
+  if (hipMalloc(&v.map, bytes) != hipSuccess) return fail(ASLHIP_ENOMEM, "ASL ERROR: visualisation fill: out of memory");
+  if (hipMemcpy(v.map, map, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(v.map);
+    return fail(ASLHIP_EHIP, "ASL ERROR: visualisation fill: map upload failed");
+  }
+  h->vis.push_back(v);
+  if (vis_id) *vis_id = static_cast<int>(h->vis.size()) - 1;
+  return 0;
+}
+
+int aslhip_vis_execute(aslhip_lattice *h, int vis_id) {
+  if (int r = check(h)) return r;
+  if (vis_id < 0 || vis_id >= (int)h->vis.size())
+    return fail(ASLHIP_EINVAL, "ASL ERROR: unknown visualisation fill id %d", vis_id);
+  if (!h->initialized) return fail(ASLHIP_ESTATE, "ASL ERROR: visualisation fill before initF");
+  // rho and u must hold the last step's values (a deferred step writes them)
+  if (int r = flush_deferred(h)) return r;
+  if (int r = order_after_consumer(h)) return r;
+  const aslhip_lattice::Vis &v = h->vis[vis_id];
+  VisArgs a;
+  a.map = v.map;
+  a.rho = h->rho;
+  a.u = h->u;
+  a.kind = v.kind;
+  a.len = h->len;
+  a.sx = h->sx;
+  a.sy = h->sy;
+  for (int d = 0; d < 3; ++d) a.u0[d] = v.u[d];
+  for (int d = 0; d < 6; ++d) a.rot[d] = v.rot[d];
+  a.has_rot = v.has_rot ? 1 : 0;
+  a.dx = h->dx;
+  for (int d = 0; d < 3; ++d) a.pos0[d] = h->pos0[d];
+  HIPCHK(launch_vis(h->Q, h->dtype, a, h->stream));
   return 0;
 }
 

@@ -7,7 +7,8 @@
 // quasiparticlesCoefs = lc_i / 6 with +1 on the rest direction
 // (src/math/aslTemplates.cxx:38-70), evaluated in double exactly as there.
 // D3Q27 has no weights in ASL (:187-200); the textbook 8/27, 2/27, 1/54,
-// 1/216 are used (extension).
+// 1/216 are used (extension).  D3Q15 and D3Q19 also carry ASL's laplacian
+// and gradient stencil coefficients (D3Q27 has none in ASL).
 #pragma once
 
 #if defined(__HIPCC__)
@@ -30,6 +31,9 @@ struct Lat<15> {
   static constexpr ASLHIP_HD double w(int i) {
     return i == 0 ? (-14. / 3.) / 6. + 1. : (i < 7 ? (2. / 3.) / 6. : (1. / 12.) / 6.);
   }
+  // laplasCoefs and gradientCoefs (src/math/aslTemplates.cxx:154-163)
+  static constexpr ASLHIP_HD double lc(int i) { return i == 0 ? -14. / 3. : (i < 7 ? 2. / 3. : 1. / 12.); }
+  static constexpr ASLHIP_HD double gc(int i) { return i == 0 ? 0. : (i < 7 ? 1. / 3. : 1. / 24.); }
 };
 
 template <>
@@ -42,6 +46,9 @@ struct Lat<19> {
   static constexpr ASLHIP_HD double w(int i) {
     return i == 0 ? -4. / 6. + 1. : (i < 7 ? (1. / 3.) / 6. : (1. / 6.) / 6.);
   }
+  // laplasCoefs and gradientCoefs (src/math/aslTemplates.cxx:177-182)
+  static constexpr ASLHIP_HD double lc(int i) { return i == 0 ? -4. : (i < 7 ? 1. / 3. : 1. / 6.); }
+  static constexpr ASLHIP_HD double gc(int i) { return i == 0 ? 0. : (i < 7 ? 1. / 6. : 1. / 12.); }
 };
 
 template <>

@@ -208,6 +208,24 @@ class Lattice:
         call("aslhip_read_f", self.h, out.ctypes.data)
         return out.reshape((self.q,) + self.S)
 
+    def add_vis(self, kind, mapv, u=None, rot=None):
+        """Register a visualisation fill of rho / u at the map's boundary nodes
+        (VIS_NOSLIP_VEL, VIS_NOSLIP_RHO, or VIS_VELOCITY with a velocity u or
+        rot = (axis, centre)); returns its id for execute_vis."""
+        m = np.ascontiguousarray(np.asarray(mapv, dtype=np.float32).reshape(self.len))
+        _, up = _d3(u if u is not None else (0.0, 0.0, 0.0))
+        rr = None
+        if rot is not None:
+            rr = np.ascontiguousarray(np.asarray(rot, dtype=np.float64).reshape(6))
+        vid = ctypes.c_int(-1)
+        call("aslhip_vis_add", self.h, int(kind), m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+             up if u is not None else None,
+             None if rr is None else rr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(vid))
+        return vid.value
+
+    def execute_vis(self, vis_id):
+        call("aslhip_vis_execute", self.h, int(vis_id))
+
     def surface_force(self, mapv):
         """ComputeSurfaceForceMap: momentum exchange at the surface of mapv, shape (3,) + S."""
         m = np.ascontiguousarray(np.asarray(mapv, dtype=np.float32).reshape(self.len))

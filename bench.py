@@ -111,12 +111,14 @@ def tgv_fields(n, u0=0.04):
     """
     k = 2.0 * np.pi / n
     x = (np.arange(n + 2, dtype=np.float64) - 1.0) * k
-    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
-    u = np.empty((3,) + X.shape)
-    u[0] = u0 * np.sin(X) * np.cos(Y) * np.cos(Z)
-    u[1] = -u0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+    s, c = np.sin(x), np.cos(x)
+    c2 = np.cos(2 * x)
+    # separable products, evaluated left to right as u0 * f(x) * g(y) * h(z)
+    u = np.empty((3, n + 2, n + 2, n + 2))
+    u[0] = (u0 * s)[:, None, None] * c[None, :, None] * c[None, None, :]
+    u[1] = (-u0 * c)[:, None, None] * s[None, :, None] * c[None, None, :]
     u[2] = 0.0
-    rho = 1.0 + 3.0 * (u0 * u0 / 16.0) * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0)
+    rho = 1.0 + (3.0 * (u0 * u0 / 16.0)) * (c2[:, None, None] + c2[None, :, None]) * (c2 + 2.0)[None, None, :]
     return rho, u
 
 

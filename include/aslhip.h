@@ -207,6 +207,34 @@ int aslhip_synchronize(aslhip_lattice *h);
 int aslhip_surface_force(aslhip_lattice *h, const float *map, void *out);
 int aslhip_surface_flux(aslhip_lattice *h, const float *map, void *out);
 
+/* Visualisation fills of the lattice's rho / u at the boundary nodes of a map
+ * (cells with map <= 0 and map > -0.9999), done on the device in place on the
+ * values the last macro-writing step left (host reads return them):
+ *   NOSLIP_VEL   generateBCNoSlipVel (src/num/aslLBGKBC.cxx:808-815) ->
+ *                BCConstantValueMiddlePointMap(velocity, 0, map)
+ *                (src/num/aslBasicBC.cxx:167-227): u(x) = -sum_i lc_i u(x+c_i)
+ *                / max(sum_i lc_i, .01) over neighbours with map > 0;
+ *   NOSLIP_RHO   generateBCNoSlipRho (:818-825) -> BCConstantGradientMap(rho,
+ *                0, map) (src/num/aslBasicBC.cxx:283-351): rho(x) = weighted
+ *                mean of rho over neighbours with map > 0, weights
+ *                (n . c_i/|c_i| + 1)^2, n the normalised map gradient; cells
+ *                whose weight sum is <= 0.1 keep their value;
+ *   VELOCITY     generateBCVelocityVel (:845-850) -> BCValuePFMap(velocity,
+ *                v, map) (src/num/aslBasicBC.cxx:124-145): u(x) = u (constant)
+ *                or axis x (position(x) - centre) (rot = {axis, centre}), at
+ *                the positions set by aslhip_set_geometry.
+ * Map reads outside the block are -0.9; field reads wrap the linear index.
+ * NOSLIP_VEL / NOSLIP_RHO need d3q15 or d3q19 (ASL's d3q27 has no stencil
+ * coefficients).  aslhip_vis_add copies the map to the device (once); each
+ * aslhip_vis_execute (an asl::NumMethod::execute of the fill) queues one
+ * kernel on the lattice's stream. */
+#define ASLHIP_VIS_NOSLIP_VEL 1
+#define ASLHIP_VIS_NOSLIP_RHO 2
+#define ASLHIP_VIS_VELOCITY 3
+int aslhip_vis_add(aslhip_lattice *h, int kind, const float *map, const double u[3], const double rot[6],
+                   int *vis_id);
+int aslhip_vis_execute(aslhip_lattice *h, int vis_id);
+
 /* asl::initData(map, DF) / generateDataContainer_SP(block, DF, gN, type) —
  * src/aslGeomInc.cxx:187-221 with the DistanceFunction tree of
  * src/math/aslDistanceFunction.cxx:55-164: the map of a distance-function

@@ -47,6 +47,8 @@ def load_library():
     lib.orc_closure_features.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     lib.orc_set_closure_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_surface.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    lib.orc_vis_fill.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.orc_vis_fill.restype = ctypes.c_int
     lib.orc_closure_features.restype = ctypes.c_int
     lib.orc_set_compute_macro.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     lib.orc_set_geometry.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
@@ -184,6 +186,17 @@ class OracleLattice:
         out = np.empty(self.len)
         self.lib.orc_surface(self.h, m.ctypes.data, 1, out.ctypes.data)
         return out.reshape(self.S)
+
+    def vis_fill(self, kind, mapv, u=None, rot=None):
+        """Visualisation fill of rho / u at the map's boundary nodes, in place:
+        kind 1 no-slip velocity, 2 no-slip density, 3 velocity u or rotation
+        rot = (axis, centre) (generateBCNoSlipVel / NoSlipRho / VelocityVel)."""
+        m = np.ascontiguousarray(np.asarray(mapv, dtype=np.float32).reshape(self.len))
+        rr = None if rot is None else np.ascontiguousarray(np.asarray(rot, dtype=np.float64).reshape(6))
+        uu = None if u is None else _vec3(u)
+        if self.lib.orc_vis_fill(self.h, int(kind), m.ctypes.data, None if uu is None else _d(uu),
+                                 None if rr is None else rr.ctypes.data) != 0:
+            raise ValueError("oracle: bad visualisation fill")
 
     def set_closure_mode(self, mode):
         """0: FP32 evaluation (bit-identical to the GPU's vector path); 1: BF16 (the matrix-core path)."""

@@ -49,6 +49,9 @@ struct Templ {
   int c[27][3];
   double w[27];  // quasiparticlesCoefs: lc_i/6, +1 on the rest direction
   int inv[27];
+  // laplasCoefs / gradientCoefs (D3Q15 and D3Q19 only: ASL's d3q27() has neither)
+  bool has_lc = false;
+  double lc[27], gc[27];
 };
 
 // VectorTemplate(int, AVec*, double* lc, double* gc): quasiparticlesCoefs[i] =
@@ -85,12 +88,20 @@ Templ make_templ(int q) {
     lc[0] = -14. / 3.;
     for (int i = 1; i < 7; ++i) lc[i] = 2. / 3.;
     for (int i = 7; i < 15; ++i) lc[i] = 1. / 12.;
+    t.gc[0] = 0.;
+    for (int i = 1; i < 7; ++i) t.gc[i] = 1. / 3.;
+    for (int i = 7; i < 15; ++i) t.gc[i] = 1. / 24.;
+    t.has_lc = true;
   } else if (q == 19) {
     t.Q = 19;
     std::memcpy(t.c, c27, sizeof(int) * 3 * 19);
     lc[0] = -4.;
     for (int i = 1; i < 7; ++i) lc[i] = 1. / 3.;
     for (int i = 7; i < 19; ++i) lc[i] = 1. / 6.;
+    t.gc[0] = 0.;
+    for (int i = 1; i < 7; ++i) t.gc[i] = 1. / 6.;
+    for (int i = 7; i < 19; ++i) t.gc[i] = 1. / 12.;
+    t.has_lc = true;
   } else if (q == 27) {
     // Extension: ASL builds d3q27() without weights.  Textbook D3Q27 weights
     // 8/27, 2/27, 1/54, 1/216, written as lc = 6 w (rest: 6 (w0 - 1)).
@@ -104,6 +115,7 @@ Templ make_templ(int q) {
     throw std::invalid_argument("lattice template must be 15, 19 or 27");
   }
   finish_templ(t, lc);
+  std::memcpy(t.lc, lc, sizeof(lc));
   if (q == 27) {  // exact textbook values (the lc/6 round trip is not exact)
     t.w[0] = 8. / 27.;
     for (int i = 1; i < 7; ++i) t.w[i] = 2. / 27.;
@@ -576,6 +588,112 @@ struct Lattice {
     return r;
   }
 
+  // Visualisation fills of rho / u at the boundary nodes of a map, in place,
+  // on cells with map <= 0 && map > -0.9999 (the kernels' outer condition).
+  // Reads come only from cells with map > 0 and writes go only to cells with
+  // map <= 0, so the in-place sweep equals ASL's one-kernel update.
+  //  kind 1, generateBCNoSlipVel (src/num/aslLBGKBC.cxx:808-815) ->
+  //    BCConstantValueMiddlePointMap(velocity, 0, map) (src/num/aslBasicBC.cxx:
+  //    167-227): over i = 1..Q-1 with b_i = map(x + c_i) > 0,
+  //      v += select(2*0 - v(x + c_i), b_i) * lc_i,  n += select(lc_i, b_i),
+  //      v(x) = v / max(n, .01);
+  //  kind 2, generateBCNoSlipRho (:818-825) -> BCConstantGradientMap(rho, 0,
+  //    map) (src/num/aslBasicBC.cxx:283-351): the map normal
+  //      g_k = sum_i map(x + c_i) gc_i c_i[k]   (gradient, src/math/aslTemplateVE.cxx:210-222)
+  //    normalised by its length (gcNormalize, aclVectorOfElementsOperations.cxx:
+  //    1213-1244), d_i = (g . c_i/|c_i| + 1)^2 and
+  //      n = max(sum_i select(d_i, b_i), .001),
+  //      r = sum_i select((rho(x + c_i) - (g . c_i/|c_i|) * 0) d_i, b_i),
+  //      rho(x) = r / n where n > 0.1 (else unchanged);
+  //  kind 3, generateBCVelocityVel (:845-850) -> BCValuePFMap(velocity, v, map)
+  //    (src/num/aslBasicBC.cxx:124-145): v(x) = the position function at the
+  //    node's physical position (Index2PositionACL of the ghost-inclusive block).
+  // Map reads outside the block are -0.9 (generateDCFullSafe(map, -.9)); field
+  // reads wrap the linear index (generateDCFullSafe(data)).  Sums run left to
+  // right in the order of the directions, dot products over the non-zero
+  // components; no operation is contracted.
+  void vis_fill(int kind, const float *mapp, const T *u, const T *rot, bool has_rot) {
+    const int Q = vt.Q;
+    if ((kind == 1 || kind == 2) && !vt.has_lc) throw std::invalid_argument("d3q27 has no laplace/gradient coefficients");
+    std::vector<float> m(mapp, mapp + len);
+    auto mx = [&](long idx) { return map_safe(m, idx, -.9f); };
+    auto wrap = [&](long idx) { return (size_t)((len + idx) % len); };
+    T chat[27][3];
+    for (int i = 0; i < Q; ++i) {
+      const double l = std::sqrt(double(vt.c[i][0] * vt.c[i][0] + vt.c[i][1] * vt.c[i][1] + vt.c[i][2] * vt.c[i][2]));
+      for (int d = 0; d < 3; ++d) chat[i][d] = i ? T(double(vt.c[i][d]) / l) : T(0);
+    }
+#pragma omp parallel for schedule(static)
+    for (long g = 0; g < len; ++g) {
+      const float mg = m[(size_t)g];
+      if (!(mg <= 0.f && mg > -.9999f)) continue;
+      if (kind == 1) {
+        T v[3] = {T(0), T(0), T(0)}, n = T(0);
+        for (int i = 1; i < Q; ++i) {
+          const bool b = mx(g + off(i)) > 0.f;
+          const size_t y = wrap(g + off(i));
+          for (int c = 0; c < 3; ++c) v[c] = v[c] + (b ? T(0) - vel[(size_t)c * len + y] : T(0)) * T(vt.lc[i]);
+          n = n + (b ? T(vt.lc[i]) : T(0));
+        }
+        const T den = n > T(.01) ? n : T(.01);
+        for (int c = 0; c < 3; ++c) vel[(size_t)c * len + g] = v[c] / den;
+      } else if (kind == 2) {
+        T gn[3];
+        for (int k = 0; k < 3; ++k) {
+          T acc = T(0);
+          bool first = true;
+          for (int i = 1; i < Q; ++i) {
+            if (vt.c[i][k] == 0) continue;
+            const T t = T(mx(g + off(i))) * T(vt.gc[i] * double(vt.c[i][k]));
+            acc = first ? t : acc + t;
+            first = false;
+          }
+          gn[k] = acc;
+        }
+        T l2 = gn[0] * gn[0];
+        l2 = l2 + gn[1] * gn[1];
+        l2 = l2 + gn[2] * gn[2];
+        const T ln = std::sqrt(l2);
+        for (int k = 0; k < 3; ++k) gn[k] = gn[k] / ln;
+        T pr[27], dc[27];
+        T n = T(0);
+        for (int i = 1; i < Q; ++i) {
+          T a = T(0);
+          bool first = true;
+          for (int d = 0; d < 3; ++d) {
+            if (vt.c[i][d] == 0) continue;
+            const T t = gn[d] * chat[i][d];
+            a = first ? t : a + t;
+            first = false;
+          }
+          pr[i] = a;
+          const T e = a + T(1.);
+          dc[i] = e * e;
+          n = n + (mx(g + off(i)) > 0.f ? dc[i] : T(0));
+        }
+        n = n > T(.001) ? n : T(.001);  // a NaN count stays below the 0.1 test either way
+        T r = T(0);
+        for (int i = 1; i < Q; ++i) {
+          const bool b = mx(g + off(i)) > 0.f;
+          r = r + (b ? (rho[wrap(g + off(i))] - pr[i] * T(0)) * dc[i] : T(0));
+        }
+        if (n > T(.1)) rho[(size_t)g] = r / n;
+      } else {
+        const long I[3] = {g / Tr[0], (g % Tr[0]) / Tr[1], (g % Tr[0]) % Tr[1]};
+        T uw[3] = {u[0], u[1], u[2]};
+        if (has_rot) {
+          T pos[3];
+          for (int d = 0; d < 3; ++d) pos[d] = T(pos0[d] - dx) + T(dx) * T(I[d]);
+          const T d0 = pos[0] - rot[3], d1 = pos[1] - rot[4], d2 = pos[2] - rot[5];
+          uw[0] = rot[1] * d2 - rot[2] * d1;
+          uw[1] = rot[2] * d0 - rot[0] * d2;
+          uw[2] = rot[0] * d1 - rot[1] * d0;
+        }
+        for (int c = 0; c < 3; ++c) vel[(size_t)c * len + g] = uw[c];
+      }
+    }
+  }
+
   void execute_bc(BC<T> &bc) {
     const int Q = vt.Q;
     switch (bc.kind) {
@@ -791,6 +909,24 @@ int orc_set_closure(void *h, const float *params, const float *mu, const float *
 
 void orc_surface(void *h, const float *map, int flux, double *out) {
   dispatch(h, [&](auto &L) { L.surface(map, flux != 0, out); });
+}
+
+// Visualisation fills (Lattice::vis_fill): kind 1 no-slip velocity, 2 no-slip
+// density, 3 velocity from a position function (u, or rot = {axis, centre}).
+int orc_vis_fill(void *h, int kind, const float *map, const double u[3], const double rot[6]) {
+  try {
+    dispatch(h, [&](auto &L) {
+      using T = std::remove_reference_t<decltype(L.rho[0])>;
+      T uu[3], rr[6];
+      for (int d = 0; d < 3; ++d) uu[d] = u ? T(u[d]) : T(0);
+      for (int d = 0; d < 6; ++d) rr[d] = rot ? T(rot[d]) : T(0);
+      L.vis_fill(kind, map, uu, rr, rot != nullptr);
+      return 0;
+    });
+  } catch (...) {
+    return -1;
+  }
+  return 0;
 }
 
 void orc_set_turbulence(void *h, int on) {

@@ -44,6 +44,7 @@ void orc_set_turbulence(void *h, int on);
 int orc_set_closure(void *h, const float *params, const float *mu, const float *sc, float out);
 /* ComputeSurfaceForceMap (flux 0: out 3 * len) / ComputeSurfaceFluxMap (flux 1: out len). */
 void orc_surface(void *h, const float *map, int flux, double *out);
+int orc_vis_fill(void *h, int kind, const float *map, const double u[3], const double rot[6]);
 /* 0: the FP32 evaluation, 1: the BF16 evaluation of the GPU's matrix cores (default). */
 void orc_set_closure_mode(void *h, int mode);
 /* Closure features (7 per interior cell) of a periodic D3Q19 lattice. */
