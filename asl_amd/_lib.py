@@ -23,6 +23,7 @@ TUNE_XCD_REMAP = 1
 TUNE_BLOCK = 2
 TUNE_GROUP_MB = 3
 TUNE_WIDE = 4
+TUNE_CPL = 5
 DF_SPHERE, DF_CYLINDER, DF_CONE, DF_PLANE, DF_AND, DF_OR, DF_NEG, DF_NORMALIZE = 1, 2, 3, 4, 5, 6, 7, 8
 CLOSURE_NPARAM = 1345
 CLOSURE_FP32, CLOSURE_BF16_MFMA = 0, 1
@@ -65,6 +66,7 @@ SIGNATURES = {
     "aslhip_template": [ctypes.c_int, _ip, _dp, _ip],
     "aslhip_batch_autotune_groups": [_vp, ctypes.c_int, _ip],
     "aslhip_get_tuning": [_vp, _ip, _ip],
+    "aslhip_tuning_value": [_vp, ctypes.c_int, _ip],
     "aslhip_df_map": [_dp, ctypes.c_int, _ip, _dp, ctypes.c_double, ctypes.c_int, _vp],
     "aslhip_stl_count": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong)],
     "aslhip_stl_read": [ctypes.c_char_p, _dp, ctypes.c_longlong],

@@ -124,48 +124,49 @@ typedef float f32x16_t __attribute__((ext_vector_type(16)));
 __device__ __forceinline__ float closure_y_mfma(const float (&x)[7], const ClosureParams &c) {
   const int l = static_cast<int>(threadIdx.x) & 63;
   const int half = l >> 5;
-  unsigned short own[8];
+  // own inputs as BF16 pairs: (x0, x1), (x2, x3), (x4, x5), (x6, 1.0 = the bias column of W1')
+  unsigned own[4];
 #pragma unroll
-  for (int j = 0; j < 7; ++j) own[j] = bf16_bits((x[j] - c.mu[j]) * c.sc[j]);
-  own[7] = 0x3f80;  // 1.0: the bias column of W1'
-  // inputs 4 half .. 4 half + 3 of the partner lane's cell (lane l ^ 32)
-  unsigned short par[4];
-  {
-    const unsigned lo = static_cast<unsigned>(own[4 * (1 - half)]) | (static_cast<unsigned>(own[4 * (1 - half) + 1]) << 16);
-    const unsigned hi = static_cast<unsigned>(own[4 * (1 - half) + 2]) | (static_cast<unsigned>(own[4 * (1 - half) + 3]) << 16);
-    const unsigned plo = static_cast<unsigned>(__shfl_xor(static_cast<int>(lo), 32));
-    const unsigned phi = static_cast<unsigned>(__shfl_xor(static_cast<int>(hi), 32));
-    par[0] = static_cast<unsigned short>(plo & 0xffffu);
-    par[1] = static_cast<unsigned short>(plo >> 16);
-    par[2] = static_cast<unsigned short>(phi & 0xffffu);
-    par[3] = static_cast<unsigned short>(phi >> 16);
+  for (int p = 0; p < 4; ++p) {
+    const unsigned lo = bf16_bits((x[2 * p] - c.mu[2 * p]) * c.sc[2 * p]);
+    const unsigned hi = p < 3 ? bf16_bits((x[2 * p + 1] - c.mu[2 * p + 1]) * c.sc[2 * p + 1]) : 0x3f80u;
+    own[p] = lo | (hi << 16);
   }
-  // weights in lane order (pack_closure)
-  bf16x4_t a1, a2[4];
-  __builtin_memcpy(&a1, &c.a1[l], 8);
-#pragma unroll
-  for (int kb = 0; kb < 4; ++kb) __builtin_memcpy(&a2[kb], &c.a2[kb][l], 8);
+  // inputs 4 half .. 4 half + 3 of the partner lane's cell (lane l ^ 32)
+  const unsigned send0 = half ? own[0] : own[2], send1 = half ? own[1] : own[3];
+  const unsigned par0 = static_cast<unsigned>(__shfl_xor(static_cast<int>(send0), 32));
+  const unsigned par1 = static_cast<unsigned>(__shfl_xor(static_cast<int>(send1), 32));
+  bf16x4_t a1;
+  __builtin_memcpy(&a1, &c.a1[l], 8);  // weights in lane order (pack_closure)
   float y[2];
 #pragma unroll
   for (int cb = 0; cb < 2; ++cb) {
     // column block cb holds cells 32 cb + col; this lane supplies inputs 4 half .. 4 half + 3
-    bf16x4_t b1;
     const bool mine = (half == cb);  // the cell 32 cb + col is this lane's own
-#pragma unroll
-    for (int j = 0; j < 4; ++j) b1[j] = static_cast<short>(mine ? own[4 * half + j] : par[j]);
+    const unsigned b0 = mine ? own[2 * half] : par0, b1w = mine ? own[2 * half + 1] : par1;
+    bf16x4_t b1;
+    b1[0] = static_cast<short>(b0 & 0xffffu);
+    b1[1] = static_cast<short>(b0 >> 16);
+    b1[2] = static_cast<short>(b1w & 0xffffu);
+    b1[3] = static_cast<short>(b1w >> 16);
     f32x16_t d1 = {};
     d1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, b1, d1, 0, 0, 0);
+    // relu and BF16 rounding of the hidden layer first: d1 is dead before the second layer starts
+    bf16x4_t hb[4];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float h = d1[4 * kb + j];
+        hb[kb][j] = static_cast<short>(bf16_bits(h > 0.f ? h : 0.f));
+      }
     f32x16_t d2;
     __builtin_memcpy(&d2, c.b2[half], 64);
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
-      bf16x4_t b2;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const float h = d1[4 * kb + j];
-        b2[j] = static_cast<short>(bf16_bits(h > 0.f ? h : 0.f));
-      }
-      d2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2[kb], b2, d2, 0, 0, 0);
+      bf16x4_t a2;
+      __builtin_memcpy(&a2, &c.a2[kb][l], 8);
+      d2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, hb[kb], d2, 0, 0, 0);
     }
     float part = 0.f;
 #pragma unroll

@@ -34,6 +34,18 @@ namespace {
 
 constexpr int BLOCK = 256;
 
+// The populations live in global memory; their pointers come from the
+// argument block in device memory, so the compiler cannot tell the address
+// space and would emit flat accesses.  These casts make them global ones.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gload(const char *p) {
+  return (const __attribute__((address_space(1))) T *)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gstore(char *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
 // ---------------------------------------------------------------------------
 // Storage codecs: FP32, FP64, and BF16 / FP16 deviations f - 1 (ASL's
 // weight-normalised populations are 1 at rest), all computed in FP32/FP64.
@@ -47,16 +59,20 @@ template <>
 struct Codec<float> {
   using C = float;
   using M = float;
-  static __device__ __forceinline__ C ld(const char *b, unsigned o) { return *reinterpret_cast<const float *>(b + o); }
-  static __device__ __forceinline__ void st(char *b, unsigned o, C v) { *reinterpret_cast<float *>(b + o) = v; }
+  static __device__ __forceinline__ C ld(const char *b, unsigned o) { return *gload<float>(b + o); }
+  static __device__ __forceinline__ void st(char *b, unsigned o, C v) { *gstore<float>(b + o) = v; }
+  static __device__ __forceinline__ C dec(float s) { return s; }
+  static __device__ __forceinline__ float enc(C v) { return v; }
 };
 
 template <>
 struct Codec<double> {
   using C = double;
   using M = double;
-  static __device__ __forceinline__ C ld(const char *b, unsigned o) { return *reinterpret_cast<const double *>(b + o); }
-  static __device__ __forceinline__ void st(char *b, unsigned o, C v) { *reinterpret_cast<double *>(b + o) = v; }
+  static __device__ __forceinline__ C ld(const char *b, unsigned o) { return *gload<double>(b + o); }
+  static __device__ __forceinline__ void st(char *b, unsigned o, C v) { *gstore<double>(b + o) = v; }
+  static __device__ __forceinline__ C dec(double s) { return s; }
+  static __device__ __forceinline__ double enc(C v) { return v; }
 };
 
 // BF16 bit pattern of f - 1.
@@ -65,11 +81,15 @@ struct Codec<unsigned short> {
   using C = float;
   using M = float;
   static __device__ __forceinline__ C ld(const char *b, unsigned o) {
-    return 1.0f + __uint_as_float(static_cast<unsigned>(*reinterpret_cast<const unsigned short *>(b + o)) << 16);
+    return 1.0f + __uint_as_float(static_cast<unsigned>(*gload<unsigned short>(b + o)) << 16);
   }
   // round to nearest even in one instruction (v_cvt_pk_bf16_f32 on gfx950)
   static __device__ __forceinline__ void st(char *b, unsigned o, C v) {
-    *reinterpret_cast<__bf16 *>(b + o) = static_cast<__bf16>(v - 1.0f);
+    *gstore<unsigned short>(b + o) = __builtin_bit_cast(unsigned short, static_cast<__bf16>(v - 1.0f));
+  }
+  static __device__ __forceinline__ C dec(unsigned short s) { return 1.0f + __uint_as_float(static_cast<unsigned>(s) << 16); }
+  static __device__ __forceinline__ unsigned short enc(C v) {
+    return __builtin_bit_cast(unsigned short, static_cast<__bf16>(v - 1.0f));
   }
 };
 
@@ -78,11 +98,13 @@ struct Codec<_Float16> {
   using C = float;
   using M = float;
   static __device__ __forceinline__ C ld(const char *b, unsigned o) {
-    return 1.0f + static_cast<float>(*reinterpret_cast<const _Float16 *>(b + o));
+    return 1.0f + static_cast<float>(*gload<_Float16>(b + o));
   }
   static __device__ __forceinline__ void st(char *b, unsigned o, C v) {
-    *reinterpret_cast<_Float16 *>(b + o) = static_cast<_Float16>(v - 1.0f);
+    *gstore<_Float16>(b + o) = static_cast<_Float16>(v - 1.0f);
   }
+  static __device__ __forceinline__ C dec(_Float16 s) { return 1.0f + static_cast<float>(s); }
+  static __device__ __forceinline__ _Float16 enc(C v) { return static_cast<_Float16>(v - 1.0f); }
 };
 
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -169,22 +191,28 @@ __device__ __forceinline__ C turbulence_w(const C (&v)[Q], C r, const C (&m)[3],
 // (4.5 a^2)/rho: a_inv(i) = -a_i exactly in IEEE arithmetic, so this is the
 // same value ASL computes twice.  On return v holds the post-collision
 // populations, r the density and m the momentum.  CL: a per-cell relaxation
-// replaces w: LBGKTurbulence's (turb), else the closure network's (D3Q19,
-// FP32 collision).
-template <int Q, class C, bool CL = false>
+// replaces w: 1 LBGKTurbulence's (turb) or the closure network's in FP32 on
+// the vector ALUs, 2 the closure network's on the matrix cores (closure:
+// D3Q19, FP32 collision).  Each is its own kernel, so that the plain and the
+// matrix-core variants do not hold the registers of the FP32 network.
+template <int Q, class C, int CL = 0>
 __device__ __forceinline__ void collide(C (&v)[Q], C w, bool compressible, bool omega, const C (&om)[3], C &r_o,
                                         C (&m_o)[3], const ClosureParams *cl = nullptr, float tau0 = 0.f,
-                                        bool cl_mfma = false, bool turb = false, C nu = C(0)) {
+                                        bool turb = false, C nu = C(0)) {
   C m[3], r;
   moments<Q, C>(v, r, m);
-  if constexpr (CL) {  // a per-cell relaxation: LBGKTurbulence or the learned closure
+  if constexpr (CL == 1) {  // LBGKTurbulence or the FP32 closure
     if (turb) {
       w = turbulence_w<Q, C>(v, r, m, compressible, nu, w);
     } else if constexpr (Q == 19 && sizeof(C) == 4) {
       float x[7];
       closure_features<Q>(v, r, m, x);
-      w = cl_mfma ? closure_relax(closure_y_mfma(x, *cl), *cl, tau0) : closure_w(x, *cl, tau0);
+      w = closure_w(x, *cl, tau0);
     }
+  } else if constexpr (CL == 2 && Q == 19 && sizeof(C) == 4) {  // the closure on the matrix cores
+    float x[7];
+    closure_features<Q>(v, r, m, x);
+    w = closure_relax(closure_y_mfma(x, *cl), *cl, tau0);
   }
   // LBGK::setOmega (src/num/aslLBGK.cxx:207-208): momentum += (2/w) omega x
   // momentum, one component after the other, each seeing the ones before it
@@ -261,7 +289,7 @@ __device__ __forceinline__ double face_val<double>(const StepArgs &a, int f, int
 // One cell of one step: chunk L of the lattice described by a.
 // PEEK: instead of colliding and storing, write the populations the cell
 // receives (after every BC link is resolved) to peek[k * len + cell].
-template <int Q, class S, bool ODD, bool PER, bool WIDE, bool CL = false, bool PEEK = false>
+template <int Q, class S, bool ODD, bool PER, bool WIDE, int CL = 0, bool PEEK = false>
 __device__ __forceinline__ void step_body(const StepArgs &a, const int L, void *peek = nullptr) {
   using CD = Codec<S>;
   using C = typename CD::C;
@@ -326,8 +354,11 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L, void *
     }
     return wn;
   };
+  // The face rule only touches wavefronts with a face cell: a wavefront-
+  // uniform branch, so interior wavefronts skip it and load plainly.
+  const bool faces = !PER && __any(fb != 0);
   unsigned bmask = 0, cmask = 0;  // bounce-back / constant links
-  if (!PER) {
+  if (faces) {
 #pragma unroll
     for (int k = 1; k < Q; ++k) {
       C unused;
@@ -342,27 +373,37 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L, void *
   // All addresses lie inside the lattice for interior cells.
   // Loads take no branch (divergent branches around loads into the same
   // register force the compiler to wait for memory inside the branch).
-  const unsigned char code = a.code[idx];
+  const unsigned char code = *gload<unsigned char>(reinterpret_cast<const char *>(a.code) + idx);
   C v[Q];
+  auto load_all = [&](unsigned bm) {
 #pragma unroll
-  for (int k = 0; k < Q; ++k) {
-    const int kb = inv_dir<Q>(k);
-    const bool bb = (bmask >> k) & 1u;
-    if constexpr (WIDE) {
-      const char *pn = a.fd[ODD ? kb : k] + (ODD ? nb(k, -1) : ib);  // normal slot
-      const char *pp = a.fd[ODD ? k : kb] + (ODD ? ib : nb(k, -1));  // the other slot of the pair
-      v[k] = CD::ld(bb ? pp : pn, 0u);
-    } else {
-      // byte offsets from the first direction array
-      const unsigned on = static_cast<unsigned>(ODD ? kb : k) * a.qs32 + (ODD ? nb(k, -1) : ib);
-      const unsigned op = static_cast<unsigned>(ODD ? k : kb) * a.qs32 + (ODD ? ib : nb(k, -1));
-      v[k] = CD::ld(a.fd[0], bb ? op : on);
+    for (int k = 0; k < Q; ++k) {
+      const int kb = inv_dir<Q>(k);
+      const bool bb = (bm >> k) & 1u;
+      if constexpr (WIDE) {
+        const char *pn = a.fd[ODD ? kb : k] + (ODD ? nb(k, -1) : ib);  // normal slot
+        const char *pp = a.fd[ODD ? k : kb] + (ODD ? ib : nb(k, -1));  // the other slot of the pair
+        v[k] = CD::ld(bb ? pp : pn, 0u);
+      } else {
+        // byte offsets from the first direction array
+        const unsigned on = static_cast<unsigned>(ODD ? kb : k) * a.qs32 + (ODD ? nb(k, -1) : ib);
+        if (bm == 0u) {
+          v[k] = CD::ld(a.fd[0], on);
+        } else {
+          const unsigned op = static_cast<unsigned>(ODD ? k : kb) * a.qs32 + (ODD ? ib : nb(k, -1));
+          v[k] = CD::ld(a.fd[0], bb ? op : on);
+        }
+      }
     }
-  }
+  };
+  if (faces)
+    load_all(bmask);
+  else
+    load_all(0u);
   // Keep the scheduler from hoisting the wait on the cell type above the
   // population loads.
   __builtin_amdgcn_sched_barrier(0);
-  if (!PER && __any(fb != 0)) {
+  if (faces) {
 #pragma unroll
     for (int k = 1; k < Q; ++k) {
       if (!(((bmask | cmask) >> k) & 1u)) continue;
@@ -379,7 +420,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L, void *
   const bool general = !PER && __any(bnd);
   unsigned skipm = 0;
   if (general && bnd) {
-    const LinkRec rec = a.rec[a.pat[idx]];
+    const LinkRec rec = a.rec[*gload<unsigned short>(reinterpret_cast<const char *>(a.pat) + 2u * idx)];
     skipm = rec.skip;
     const C *val = static_cast<const C *>(a.val);
 #pragma unroll
@@ -418,8 +459,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L, void *
     for (int d = 0; d < 3; ++d) om[d] = sizeof(C) == 4 ? C(a.om32[d]) : C(a.om64[d]);
   }
   collide<Q, C, CL>(v, C(a.w), (a.flags & STEP_F_COMPRESSIBLE) != 0, omega, om, r, m, a.cl, a.cl_tau0,
-                    (a.flags & STEP_F_CLOSURE_MFMA) != 0, (a.flags & STEP_F_TURB) != 0,
-                    sizeof(C) == 4 ? C(a.nu32) : C(a.nu64));
+                    (a.flags & STEP_F_TURB) != 0, sizeof(C) == 4 ? C(a.nu32) : C(a.nu64));
 
   if (!live || !act) return;
   // Stores address every direction from fd[0] too (one scalar base pair
@@ -440,12 +480,224 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L, void *
 #pragma unroll
     for (int k = 0; k < Q; ++k) st(k);
   }
-  if (a.flags & STEP_F_RHO) static_cast<M *>(a.rho)[idx] = M(r);
+  if (a.flags & STEP_F_RHO) *gstore<M>(static_cast<char *>(a.rho) + sizeof(M) * idx) = M(r);
   if (a.flags & STEP_F_VEL) {
     const bool comp = (a.flags & STEP_F_COMPRESSIBLE) != 0;
-    M *u = static_cast<M *>(a.u);
+    char *u = static_cast<char *>(a.u);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) u[d * a.len + idx] = M(comp ? m[d] / r : m[d]);
+    for (int d = 0; d < 3; ++d)
+      *gstore<M>(u + sizeof(M) * (static_cast<size_t>(d) * a.len + idx)) = M(comp ? m[d] / r : m[d]);
+  }
+}
+
+// CPL consecutive cells along z in one lane (a vector of raw storage
+// elements; only element alignment is assumed, gfx950 loads and stores such
+// vectors as one access).
+template <class S, int CPL>
+struct RunVec {
+  typedef S type __attribute__((ext_vector_type(CPL), aligned(sizeof(S))));
+};
+
+// One step of CPL consecutive cells z0 .. z0 + CPL - 1 of a row (lanes cover
+// the interior only; nz is a multiple of CPL), non-periodic lattices under
+// 4 GiB, plain or rotating-frame LBGK.  Same addresses, same collision and
+// same link resolution per cell as step_body, so every cell ends bit for bit
+// where step_body leaves it; each direction moves CPL elements per load and
+// store.  Cells on a face layer or with a link pattern are fixed up after the
+// vector loads (wavefront-uniform branches), and a run with a cell that is
+// not computed, or that must not write some links, stores cell by cell.
+template <int Q, class S, bool ODD, int CPL>
+__device__ __forceinline__ void step_body_run(const StepArgs &a, const int L) {
+  using CD = Codec<S>;
+  using C = typename CD::C;
+  using M = typename CD::M;
+  using V = typename RunVec<S, CPL>::type;
+  constexpr unsigned ES = sizeof(S);
+  const int plane = L / a.bpp;
+  const int chunk = L - plane * a.bpp;
+  const unsigned j = static_cast<unsigned>(chunk * a.block) + threadIdx.x;
+  if (j >= static_cast<unsigned>(a.ny * a.nzv)) return;
+  // row and run inside the plane (exact: j * nzv < 2^40, see StepArgs::vmagic)
+  const unsigned row = static_cast<unsigned>((static_cast<unsigned long long>(j) * a.vmagic) >> 40);
+  const int z0 = 1 + CPL * static_cast<int>(j - row * static_cast<unsigned>(a.nzv));
+  const int x = plane + 1, y = static_cast<int>(row) + 1;
+  const unsigned idx = static_cast<unsigned>(x * a.sx + y * a.sy + z0);
+  const unsigned ib = idx * ES;
+  const char *f0 = a.fd[0];
+
+  // Face rule (as step_body), in one round of loads.  A run lies on an x or y
+  // face with all its cells, so a bounce-back link there is a vector load of
+  // the other slot of the link pair; only the first (z = 1) and the last
+  // (z = nz) cell of a row lie on a z face, and their links through it come
+  // from side loads of one element, issued with the vector loads.
+  const bool raw_links = (a.flags & STEP_F_RAW) != 0;
+  const unsigned fxy = raw_links ? 0u
+                                 : (x == 1 ? 1u : 0u) | (x == a.nx ? 2u : 0u) | (y == 1 ? 4u : 0u) |
+                                       (y == a.ny ? 8u : 0u);
+  const bool zlo = !raw_links && z0 == 1, zhi = !raw_links && z0 + CPL - 1 == a.nz;
+  const bool xyf = __any(fxy != 0u), zf = __any(zlo || zhi);  // wavefront-uniform
+  // winning face BC of link k among the faces in fb: slot * 4 + mode, and its value
+  auto winner = [&](int k, unsigned fb, C &val) -> int {
+    int wn = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int cd = Lat<Q>::c[k][d];
+      if (cd == 0) continue;
+      const int fc = 2 * d + (cd > 0 ? 0 : 1);  // the face this link enters through
+      const int cand = ((fb >> fc) & 1u) ? a.fface[fc] : 0;
+      if (cand > wn) {
+        wn = cand;
+        val = face_val<C>(a, fc, k);
+      }
+    }
+    return wn;
+  };
+  unsigned bxy = 0, cxy = 0;  // bounce-back / constant links of the x and y faces
+  if (xyf) {
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+      C unused;
+      const int md = winner(k, fxy, unused) & 3;
+      if (md == MODE_BOUNCE) bxy |= 1u << k;
+      if (md == MODE_CONST) cxy |= 1u << k;
+    }
+  }
+  // the other slot of link k's pair for the cell at byte offset ibc
+  auto other = [&](int k, unsigned ibc) -> unsigned {
+    const unsigned o = ES * static_cast<unsigned>(offc<Q>(k, a.sx, a.sy));
+    return static_cast<unsigned>(ODD ? k : inv_dir<Q>(k)) * a.qs32 + (ODD ? ibc : ibc - o);
+  };
+  V raw[Q];
+  C side[Q];  // links through z = 1 (first cell) or z = nz (last cell): the other slot
+#pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    // normal slot: (x, k) (even) or (x - c_k, inv k) (odd)
+    const unsigned o = ES * static_cast<unsigned>(offc<Q>(k, a.sx, a.sy));
+    const unsigned on = static_cast<unsigned>(ODD ? inv_dir<Q>(k) : k) * a.qs32 + (ODD ? ib - o : ib);
+    if (xyf) {
+      const unsigned op = other(k, ib);
+      raw[k] = *gload<V>(f0 + (((bxy >> k) & 1u) ? op : on));
+    } else {
+      raw[k] = *gload<V>(f0 + on);
+    }
+    if (zf && Lat<Q>::c[k][2] != 0) side[k] = CD::ld(f0, other(k, Lat<Q>::c[k][2] > 0 ? ib : ib + ES * (CPL - 1)));
+  }
+  unsigned codes = 0;  // CPL cell-type bytes
+  const char *cb = reinterpret_cast<const char *>(a.code) + idx;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) codes |= static_cast<unsigned>(*gload<unsigned char>(cb + c)) << (8 * c);
+  __builtin_amdgcn_sched_barrier(0);
+  C v[CPL][Q];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c)
+#pragma unroll
+    for (int k = 0; k < Q; ++k) v[c][k] = CD::dec(raw[k][c]);
+  if (xyf) {
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+      if (!(((bxy | cxy) >> k) & 1u)) continue;
+      C val = C(0);
+      (void)winner(k, fxy, val);
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) v[c][k] = ((bxy >> k) & 1u) ? v[c][k] + val : val;
+    }
+  }
+  if (zf) {
+    // a link through the z face of an end cell: the z face's BC where it wins
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+      const int cz = Lat<Q>::c[k][2];
+      if (cz == 0) continue;
+      const int c = cz > 0 ? 0 : CPL - 1;
+      const bool on_z = cz > 0 ? zlo : zhi;
+      const unsigned zbit = cz > 0 ? 16u : 32u;
+      C vz = C(0), vxy = C(0);
+      const int wz = winner(k, fxy | zbit, vz);
+      const int wxy = winner(k, fxy, vxy);
+      if (on_z && wz != wxy) {  // the z face wins the link
+        v[c][k] = (wz & 3) == MODE_BOUNCE ? side[k] + vz : vz;
+      }
+    }
+  }
+
+  // Cells with a link pattern re-read their links from it.
+  bool live[CPL];
+  bool bnd_any = false;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    const unsigned char code = static_cast<unsigned char>(codes >> (8 * c));
+    live[c] = code != CODE_SKIP;
+    bnd_any |= live[c] && code != CODE_FLUID;
+  }
+  unsigned skipm[CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) skipm[c] = 0u;
+  const bool general = __any(bnd_any);
+  if (general && bnd_any) {
+    const C *val = static_cast<const C *>(a.val);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      const unsigned char code = static_cast<unsigned char>(codes >> (8 * c));
+      if (!live[c] || code == CODE_FLUID) continue;
+      const unsigned ibc = ib + ES * c;
+      const LinkRec rec = a.rec[a.pat[idx + c]];
+      skipm[c] = rec.skip;
+#pragma unroll
+      for (int k = 1; k < Q; ++k) {
+        const int slot = raw_links ? 0 : rec.slot[k];
+        const unsigned char md = slot ? a.mode[slot] : static_cast<unsigned char>(MODE_NONE);
+        const unsigned o = ES * static_cast<unsigned>(offc<Q>(k, a.sx, a.sy));
+        const unsigned on = static_cast<unsigned>(ODD ? inv_dir<Q>(k) : k) * a.qs32 + (ODD ? ibc - o : ibc);
+        const unsigned op = static_cast<unsigned>(ODD ? k : inv_dir<Q>(k)) * a.qs32 + (ODD ? ibc : ibc - o);
+        const C xk = CD::ld(f0, md == MODE_BOUNCE ? op : on);
+        const C sv = val[slot * Q + k];
+        v[c][k] = md == MODE_BOUNCE ? xk + sv : (md == MODE_CONST ? sv : xk);
+      }
+    }
+  }
+
+  const bool omega = (a.flags & STEP_F_OMEGA) != 0;
+  const bool comp = (a.flags & STEP_F_COMPRESSIBLE) != 0;
+  C om[3] = {C(0), C(0), C(0)};
+  if (omega) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) om[d] = sizeof(C) == 4 ? C(a.om32[d]) : C(a.om64[d]);
+  }
+  C r[CPL], m[CPL][3];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) collide<Q, C>(v[c], C(a.w), comp, omega, om, r[c], m[c]);
+
+  bool plain = true;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) plain = plain && live[c] && skipm[c] == 0u;
+  char *fw = a.fd[0];
+#pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    // even: slot (x, inv k); odd: slot (x + c_k, k)
+    const unsigned o = ES * static_cast<unsigned>(offc<Q>(k, a.sx, a.sy));
+    const unsigned ow = static_cast<unsigned>(ODD ? k : inv_dir<Q>(k)) * a.qs32 + (ODD ? ib + o : ib);
+    if (plain) {
+      V out;
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) out[c] = CD::enc(v[c][k]);
+      *gstore<V>(fw + ow) = out;
+    } else {
+#pragma unroll
+      for (int c = 0; c < CPL; ++c)
+        if (live[c] && !((skipm[c] >> k) & 1u)) CD::st(fw, ow + ES * c, v[c][k]);
+    }
+  }
+  if (a.flags & (STEP_F_RHO | STEP_F_VEL)) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      if (!live[c]) continue;
+      if (a.flags & STEP_F_RHO) static_cast<M *>(a.rho)[idx + c] = M(r[c]);
+      if (a.flags & STEP_F_VEL) {
+        M *u = static_cast<M *>(a.u);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) u[d * a.len + idx + c] = M(comp ? m[c][d] / r[c] : m[c][d]);
+      }
+    }
   }
 }
 
@@ -460,17 +712,36 @@ __device__ __forceinline__ void step_body(const StepArgs &a, const int L, void *
 // one ended, on the populations written last, which its XCD's L2 still
 // holds.  Every cell's reads and writes in one step are its own (AA
 // pattern), so the order of the chunks does not change results.
-template <int Q, class S, bool ODD, bool PER, bool WIDE, bool CL>
+template <int Q, class S, bool ODD, bool PER, bool WIDE, int CL>
 __global__ __launch_bounds__(STEP_BLOCK) void k_step(const StepArgs *args) {
   const StepArgs &a = *args;
   const int b = ODD ? xcd_reverse(static_cast<int>(blockIdx.x), a.nblocks) : static_cast<int>(blockIdx.x);
   step_body<Q, S, ODD, PER, WIDE, CL>(a, a.remap > 1 ? xcd_remap(b, a.nblocks, a.remap) : b);
 }
 
+// The matrix-core closure variant, held to 5 waves per SIMD (96 VGPRs).
+template <int Q, class S, bool ODD, bool PER, bool WIDE>
+__global__ __launch_bounds__(STEP_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_step_mfma(
+    const StepArgs *args) {
+  const StepArgs &a = *args;
+  const int b = ODD ? xcd_reverse(static_cast<int>(blockIdx.x), a.nblocks) : static_cast<int>(blockIdx.x);
+  step_body<Q, S, ODD, PER, WIDE, 2>(a, a.remap > 1 ? xcd_remap(b, a.nblocks, a.remap) : b);
+}
+
+// The step with CPL cells per lane (step_body_run); same chunk order.  At
+// most STEP_RUN_BLOCK lanes per workgroup: four cells per lane hold about
+// 130 VGPRs, more than a 1024-lane workgroup allows without spilling.
+template <int Q, class S, bool ODD, int CPL>
+__global__ __launch_bounds__(STEP_RUN_BLOCK) void k_step_run(const StepArgs *args) {
+  const StepArgs &a = *args;
+  const int b = ODD ? xcd_reverse(static_cast<int>(blockIdx.x), a.nblocks) : static_cast<int>(blockIdx.x);
+  step_body_run<Q, S, ODD, CPL>(a, a.remap > 1 ? xcd_remap(b, a.nblocks, a.remap) : b);
+}
+
 // The incoming populations of every computed cell for the next step.
 template <int Q, class S, bool ODD, bool PER>
 __global__ __launch_bounds__(STEP_BLOCK) void k_peek(const StepArgs *args, void *out) {
-  step_body<Q, S, ODD, PER, false, false, true>(*args, static_cast<int>(blockIdx.x), out);
+  step_body<Q, S, ODD, PER, false, 0, true>(*args, static_cast<int>(blockIdx.x), out);
 }
 
 // asl::ComputeSurfaceForceMap / ComputeSurfaceFluxMap (src/num/aslLBGKBC.cxx:
@@ -896,17 +1167,38 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_final(const double *partial, i
 template <int Q, class S, bool ODD, bool PER>
 hipError_t step_w(bool wide, const StepLaunch &a, hipStream_t s) {
   const dim3 grid(a.nblocks), block(a.block);
-  if (a.closure) {  // the per-cell relaxation variant (closure: D3Q19 FP32 only, checked by the host)
-    if (wide)
-      hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true, true>), grid, block, 0, s, a.args);
+  if (a.cpl > 1) {
+    if (PER || wide || a.variant) return hipErrorInvalidValue;  // the host sends these one cell per lane
+    if (a.cpl == 2)
+      hipLaunchKernelGGL((k_step_run<Q, S, ODD, 2>), grid, block, 0, s, a.args);
+    else if (a.cpl == 4)
+      hipLaunchKernelGGL((k_step_run<Q, S, ODD, 4>), grid, block, 0, s, a.args);
     else
-      hipLaunchKernelGGL((k_step<Q, S, ODD, PER, false, true>), grid, block, 0, s, a.args);
+      return hipErrorInvalidValue;
+    return hipGetLastError();
+  }
+  if (a.variant == 2) {  // the closure on the matrix cores: D3Q19 with FP32 collision (checked by the host)
+    if constexpr (Q == 19 && sizeof(typename Codec<S>::C) == 4) {
+      if (wide)
+        hipLaunchKernelGGL((k_step_mfma<Q, S, ODD, PER, true>), grid, block, 0, s, a.args);
+      else
+        hipLaunchKernelGGL((k_step_mfma<Q, S, ODD, PER, false>), grid, block, 0, s, a.args);
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
+  if (a.variant == 1) {  // LBGKTurbulence or the FP32 closure
+    if (wide)
+      hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true, 1>), grid, block, 0, s, a.args);
+    else
+      hipLaunchKernelGGL((k_step<Q, S, ODD, PER, false, 1>), grid, block, 0, s, a.args);
     return hipGetLastError();
   }
   if (wide)
-    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true, false>), grid, block, 0, s, a.args);
+    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, true, 0>), grid, block, 0, s, a.args);
   else
-    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, false, false>), grid, block, 0, s, a.args);
+    hipLaunchKernelGGL((k_step<Q, S, ODD, PER, false, 0>), grid, block, 0, s, a.args);
   return hipGetLastError();
 }
 

@@ -7,8 +7,10 @@
 
 namespace aslhip {
 
-// Threads per workgroup of the step kernel (each thread one cell).
+// Threads per workgroup of the step kernel (each thread one cell), at most;
+// of its variant with runs of 2 or 4 cells per lane, at most.
 constexpr int STEP_BLOCK = 1024;
+constexpr int STEP_RUN_BLOCK = 512;
 
 // Step-kernel flags.
 enum : unsigned {
@@ -24,8 +26,10 @@ enum : unsigned {
 };
 
 // One fused collide+stream pass (AA pattern) over the ghost-inclusive block.
-// The launch is a 1-D grid of STEP_BLOCK-cell chunks of the interior planes
-// x = 1..nx (rows y = 1..ny, every z including the two ghost columns).
+// The launch is a 1-D grid of chunks of `block` lanes of the interior planes
+// x = 1..nx: with cpl = 1 a lane is one cell of rows y = 1..ny, every z
+// including the two ghost columns; with cpl = 2 or 4 a lane is a run of cpl
+// interior cells z0 .. z0 + cpl - 1 (nz a multiple of cpl).
 //
 // Links into cells of the six face layers are resolved from the cell's
 // coordinates: for each face, fface[f] = slot * 4 + mode of the last
@@ -46,6 +50,9 @@ struct StepArgs {
   int nblocks;                // bpp * nx
   unsigned long long zmagic;  // ceil(2^40 / sy): row = (off * zmagic) >> 40
   int remap;                  // > 1: runs of that many chunks per XCD (see xcd_remap)
+  int cpl;                    // cells per lane along z (1: one cell per lane, ghost columns included)
+  int nzv;                    // runs per row, nz / cpl (cpl > 1)
+  unsigned long long vmagic;  // ceil(2^40 / nzv): row = (lane * vmagic) >> 40
   int fface[6];               // per face: slot * 4 + mode of its winning slice BC
   float fval32[6][27];        // per face and direction: addend / constant (FP32 lattices)
   double fval64[6][27];       // the same for FP64 lattices
@@ -173,7 +180,8 @@ struct StepLaunch {
   const StepArgs *args;
   int nblocks;
   int block;
-  bool closure;  // the per-cell relaxation variant (closure or LBGKTurbulence)
+  int variant;   // per-cell relaxation: 0 none, 1 LBGKTurbulence or the FP32 closure, 2 the matrix-core closure
+  int cpl;       // cells per lane (2 or 4: the run variant; non-periodic, under 4 GiB, no closure)
 };
 
 // wide: the populations span more than 4 GiB (64-bit per-lane addresses).

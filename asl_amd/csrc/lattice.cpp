@@ -184,6 +184,7 @@ struct aslhip_lattice {
   int tune_block = 512;   // workgroup size (measured best with runs of 4 chunks per XCD)
   bool force_wide = false;  // 64-bit addressing on any size (ASLHIP_TUNE_WIDE)
   int tune_remap = 4;  // chunk runs per XCD (measured best for graph chunks; the autotuner may choose others)
+  int tune_cpl = 0;    // cells per lane along z: 0 automatic, else 1, 2 or 4 (ASLHIP_TUNE_CPL)
   // face rule: per face slot * 4 + mode of the winning slice BC, and values
   int fface[6] = {0, 0, 0, 0, 0, 0};
   float fval32[6][27] = {};
@@ -601,8 +602,25 @@ int run_vlinks(aslhip_lattice *h, bool after_odd) {
 
 bool is_wide(const aslhip_lattice *h) { return h->force_wide || h->qs_bytes * h->Q > 0xffffffffLL; }
 
+// Cells per lane the step kernel runs with: runs of 2 or 4 cells along z on
+// non-periodic lattices under 4 GiB without a per-cell relaxation, when nz is
+// a multiple of the run; else one cell per lane (the longest run that divides
+// nz).  Automatic: one cell per lane, measured fastest on the sphere channel.
+int effective_cpl(const aslhip_lattice *h) {
+  if (h->periodic || is_wide(h) || h->has_closure || h->turbulence) return 1;
+  int c = h->tune_cpl;
+  if (c == 0) c = 1;
+  while (c > 1 && h->n[2] % c) c /= 2;
+  return c;
+}
+
+// Workgroup size the step kernel runs with (runs of cells: at most STEP_RUN_BLOCK).
+int step_workgroup(const aslhip_lattice *h) {
+  return effective_cpl(h) > 1 && h->tune_block > STEP_RUN_BLOCK ? STEP_RUN_BLOCK : h->tune_block;
+}
+
 // Kernel arguments of one step of h (flags filled in by the caller).
-StepArgs step_args(const aslhip_lattice *h, int block = 0) {
+StepArgs step_args(const aslhip_lattice *h, int block = 0, int cpl = 0) {
   StepArgs a;
   std::memset(&a, 0, sizeof a);  // compared bytewise by upload_args
   for (int k = 0; k < 27; ++k) a.fd[k] = k < h->Q ? h->f + k * h->qs_bytes : nullptr;
@@ -620,10 +638,16 @@ StepArgs step_args(const aslhip_lattice *h, int block = 0) {
   a.ny = h->n[1];
   a.nz = h->n[2];
   a.len = h->len;
+  a.cpl = cpl > 0 ? cpl : effective_cpl(h);
   a.block = block > 0 ? block : h->tune_block;
-  a.bpp = (h->n[1] * h->sy + a.block - 1) / a.block;
+  if (a.cpl > 1 && a.block > STEP_RUN_BLOCK) a.block = STEP_RUN_BLOCK;
+  a.nzv = h->n[2] / a.cpl;
+  // lanes per plane: every cell of the rows (ghost columns included), or the runs of the interior
+  const int lanes = a.cpl > 1 ? h->n[1] * a.nzv : h->n[1] * h->sy;
+  a.bpp = (lanes + a.block - 1) / a.block;
   a.nblocks = a.bpp * h->n[0];
   a.zmagic = ((1ULL << 40) + (unsigned long long)h->sy - 1) / (unsigned long long)h->sy;
+  a.vmagic = ((1ULL << 40) + (unsigned long long)a.nzv - 1) / (unsigned long long)a.nzv;
   a.remap = h->tune_remap;
   a.code = h->code;
   a.pat = h->pat;
@@ -682,7 +706,9 @@ int launch_one(aslhip_lattice *h, unsigned fl, std::vector<hipEvent_t> *ev, int 
   a.nblocks = h->h_args[0].nblocks;
   a.block = h->h_args[0].block;
   a.args = h->d_args + fl;
-  a.closure = h->has_closure || h->turbulence;
+  // LBGKTurbulence takes precedence over a closure in the collision
+  a.variant = h->turbulence ? 1 : (h->has_closure ? (h->closure_mfma ? 2 : 1) : 0);
+  a.cpl = h->h_args[0].cpl;
   if (ev) HIPCHK(hipEventRecord((*ev)[2 * s], h->stream));
   HIPCHK(launch_step(h->Q, h->dtype, h->next_odd, h->periodic, is_wide(h), a, h->stream));
   if (ev) HIPCHK(hipEventRecord((*ev)[2 * s + 1], h->stream));
@@ -1356,11 +1382,16 @@ int surface_impl(aslhip_lattice *h, const float *map, void *out, bool flux) {
   const size_t outb = (size_t)h->len * (flux ? 1 : 3) * h->melem;
   void *post = nullptr, *in = nullptr, *dout = nullptr;
   float *dmap = nullptr;
+  StepArgs *dpk = nullptr;  // the step arguments with one cell per lane, for the peek
+  StepArgs pk = step_args(h, 0, 1);
+  pk.flags = base_flags(h) & ~(STEP_F_CLOSURE | STEP_F_CLOSURE_MFMA);
   hipError_t e = hipMalloc(&post, view);
   if (e == hipSuccess) e = hipMalloc(&in, view);
   if (e == hipSuccess) e = hipMalloc(&dout, outb);
   if (e == hipSuccess) e = hipMalloc(&dmap, (size_t)h->len * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dpk, sizeof(StepArgs));
   int r = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(dpk, &pk, sizeof(StepArgs), hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(dmap, map, (size_t)h->len * sizeof(float), hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess) e = hipMemsetAsync(in, 0, view, h->stream);
   if (e == hipSuccess) {
@@ -1382,10 +1413,11 @@ int surface_impl(aslhip_lattice *h, const float *map, void *out, bool flux) {
   }
   if (e == hipSuccess) {
     StepLaunch a;
-    a.nblocks = h->h_args[0].nblocks;
-    a.block = h->h_args[0].block;
-    a.args = h->d_args + (base_flags(h) & ~(STEP_F_CLOSURE | STEP_F_CLOSURE_MFMA));
-    a.closure = false;
+    a.nblocks = pk.nblocks;
+    a.block = pk.block;
+    a.args = dpk;
+    a.variant = 0;
+    a.cpl = 1;
     e = launch_peek(h->Q, h->dtype, h->next_odd, h->periodic, a, in, h->stream);
   }
   if (e == hipSuccess) {
@@ -1403,7 +1435,7 @@ int surface_impl(aslhip_lattice *h, const float *map, void *out, bool flux) {
   if (e == hipSuccess) r = copy_out(h, dout, out, outb);
   else r = fail(ASLHIP_EHIP, "ASL ERROR: surface force/flux: %s", hipGetErrorString(e));
   (void)hipStreamSynchronize(h->stream);
-  void *ptrs[] = {post, in, dout, dmap};
+  void *ptrs[] = {post, in, dout, dmap, dpk};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   return r;
@@ -1465,7 +1497,7 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]) {
 
 int aslhip_get_tuning(aslhip_lattice *h, int *block, int *remap) {
   if (int r = check(h)) return r;
-  if (block) *block = h->tune_block;
+  if (block) *block = step_workgroup(h);
   if (remap) *remap = h->tune_remap;
   return 0;
 }
@@ -1484,6 +1516,14 @@ int aslhip_set_tuning(aslhip_lattice *h, int knob, int value) {
     h->tune_remap = value;
     return 0;
   }
+  if (knob == ASLHIP_TUNE_CPL) {
+    if (value != 0 && value != 1 && value != 2 && value != 4)
+      return fail(ASLHIP_EINVAL, "ASL ERROR: cells per lane %d is not 0 (automatic), 1, 2 or 4", value);
+    if (int r = flush_deferred(h)) return r;
+    if (value != h->tune_cpl) drop_graphs(h);  // captured launches name the variant and grid
+    h->tune_cpl = value;
+    return 0;
+  }
   if (knob == ASLHIP_TUNE_BLOCK) {
     if (value < 64 || value > STEP_BLOCK || value % 64)
       return fail(ASLHIP_EINVAL, "ASL ERROR: workgroup size %d is not a multiple of 64 in 64..%d", value, STEP_BLOCK);
@@ -1495,28 +1535,49 @@ int aslhip_set_tuning(aslhip_lattice *h, int knob, int value) {
   return fail(ASLHIP_EINVAL, "ASL ERROR: unknown tuning knob %d", knob);
 }
 
+int aslhip_tuning_value(aslhip_lattice *h, int knob, int *value) {
+  if (int r = check(h)) return r;
+  if (!value) return fail(ASLHIP_EINVAL, "ASL ERROR: tuning_value needs an output");
+  switch (knob) {
+    case ASLHIP_TUNE_XCD_REMAP: *value = h->tune_remap; return 0;
+    case ASLHIP_TUNE_BLOCK: *value = step_workgroup(h); return 0;
+    case ASLHIP_TUNE_WIDE: *value = is_wide(h) ? 1 : 0; return 0;
+    case ASLHIP_TUNE_CPL: *value = effective_cpl(h); return 0;
+    default: return fail(ASLHIP_EINVAL, "ASL ERROR: unknown tuning knob %d", knob);
+  }
+}
+
 int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_remap) {
   if (int r = check(h)) return r;
   if (nsteps < 1) return fail(ASLHIP_EINVAL, "ASL ERROR: autotune needs at least one step per candidate");
   nsteps += nsteps & 1;
   const int blocks[3] = {256, 512, 1024};
   double best = 1e300;
-  int bb = h->tune_block, br = h->tune_remap;
-  for (int remap : {0, 2, 4, 8, 16, 64})
-    for (int blk : blocks) {
-      if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_XCD_REMAP, remap)) return r;
-      if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_BLOCK, blk)) return r;
-      double tot = 0., ker = 0.;
-      if (int r = aslhip_time_steps(h, nsteps, 0u, &tot, &ker, nullptr)) return r;
-      if (ker < best) {
-        best = ker;
-        bb = blk;
-        br = remap;
+  int bb = h->tune_block, br = h->tune_remap, bc = h->tune_cpl;
+  std::vector<int> cpls;  // the distinct cells-per-lane values this lattice can run
+  for (int c : {1, 2, 4}) {
+    if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_CPL, c)) return r;
+    if (effective_cpl(h) == c) cpls.push_back(c);
+  }
+  for (int cpl : cpls)
+    for (int remap : {0, 2, 4, 8, 16, 64})
+      for (int blk : blocks) {
+        if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_CPL, cpl)) return r;
+        if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_XCD_REMAP, remap)) return r;
+        if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_BLOCK, blk)) return r;
+        double tot = 0., ker = 0.;
+        if (int r = aslhip_time_steps(h, nsteps, 0u, &tot, &ker, nullptr)) return r;
+        if (ker < best) {
+          best = ker;
+          bb = blk;
+          br = remap;
+          bc = cpl;
+        }
       }
-    }
+  if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_CPL, bc)) return r;
   if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_XCD_REMAP, br)) return r;
   if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_BLOCK, bb)) return r;
-  if (best_block) *best_block = bb;
+  if (best_block) *best_block = step_workgroup(h);
   if (best_remap) *best_remap = br;
   return 0;
 }
@@ -1627,7 +1688,7 @@ int batch_tables(aslhip_batch *b) {
       gbytes = 0.;
     }
     gbytes += bytes;
-    const StepArgs a = step_args(b->lat[l], b->block);
+    const StepArgs a = step_args(b->lat[l], b->block, 1);
     for (int c = 0; c < a.nblocks; ++c) ent.push_back(make_uint2(static_cast<unsigned>(l), static_cast<unsigned>(c)));
   }
   b->groups.emplace_back(static_cast<int>(gfirst), static_cast<int>(ent.size()));
@@ -1700,7 +1761,7 @@ int batch_step_impl(aslhip_batch *b, int nsteps, unsigned flags) {
       return fail(ASLHIP_EINVAL, "ASL ERROR: per-cell relaxation (closure, turbulence) is not supported in batches");
     if (h->next_odd != odd) return fail(ASLHIP_ESTATE, "ASL ERROR: batch lattices are at different step parities");
     if (int r = order_after_consumer(h)) return r;
-    desc[l] = step_args(h, b->block);
+    desc[l] = step_args(h, b->block, 1);
     desc[n + l] = desc[l];
     if (h->comp_rho) desc[n + l].flags |= STEP_F_RHO;
     if (h->comp_vel) desc[n + l].flags |= STEP_F_VEL;

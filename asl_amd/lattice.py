@@ -164,6 +164,12 @@ class Lattice:
         call("aslhip_get_tuning", self.h, ctypes.byref(b), ctypes.byref(r))
         return b.value, r.value
 
+    def tuning_value(self, knob):
+        """The value a tuning knob has now (TUNE_CPL: the cells per lane the step kernel runs with)."""
+        v = ctypes.c_int(0)
+        call("aslhip_tuning_value", self.h, int(knob), ctypes.byref(v))
+        return v.value
+
     def autotune(self, nsteps=20):
         """Time every launch setting over nsteps real steps and keep the fastest: (block, remap)."""
         b, r = ctypes.c_int(0), ctypes.c_int(0)

@@ -38,8 +38,9 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--autotun
            (D3Q27 periodic Taylor-Green vortex on N^3, tau = 0.5005: FP32, FP16
            and BF16 storage against FP64; one JSON line per storage type)
        python bench.py --closure-study N [--tau-fine T] [--chunk C] [--train-snapshots K]
-           [--eval-chunks E]  (learned closure: N^3 coarse vs (4N)^3 fine decaying
-           turbulence; KE(t) of fine, coarse and coarse+closure, wall times)
+           [--eval-chunks E] [--closure-target total|filtered]  (learned closure: N^3
+           coarse vs (4N)^3 fine decaying turbulence; KE(t) of fine, coarse and
+           coarse+closure, wall times)
        python bench.py --cavity-study   (the 64^3 lid-driven cavity: GPU and CPU-oracle MLUPS)
        python bench.py --storage-study  (the sphere channel with FP32, FP16, BF16, FP64 storage)
        python bench.py --batch-study [--steps K]
@@ -354,6 +355,35 @@ def storage_study(steps=2000):
         del L
 
 
+def launch_study():
+    """Step-kernel duration (HIP events, median of 200 launches) on the sphere
+    channel for cells per lane x XCD run length x workgroup size."""
+    from asl_amd import _lib
+    mapv, u = setup_fields()
+    L = build_gpu(mapv, u)
+    L.step(20, write_macro=False)
+    counted = L.stats()["counted"]
+    byts = 152.0 * counted + 1.0 * int(np.prod(SIZE))
+    for cpl in (1, 2, 4):
+        for remap in (0, 4, 16, 64):
+            for block in (256, 512, 1024):
+                if cpl > 1 and block > 512:
+                    continue
+                L.set_tuning(_lib.TUNE_CPL, cpl)
+                L.set_tuning(_lib.TUNE_XCD_REMAP, remap)
+                L.set_tuning(_lib.TUNE_BLOCK, block)
+                L.time_steps(20, write_macro=False)
+                _, _, each = L.time_steps(200, write_macro=False, each=True)
+                med = float(np.median(each))
+                print(json.dumps({"study": "step-kernel launch settings, 256x128x128 sphere channel",
+                                  "cells_per_lane": L.tuning_value(_lib.TUNE_CPL), "xcd_run": remap,
+                                  "workgroup": block, "kernel_ms_median": round(med, 5),
+                                  "kernel_ms_even": round(float(np.median(each[0::2])), 5),
+                                  "kernel_ms_odd": round(float(np.median(each[1::2])), 5),
+                                  "tbps_algorithmic": round(byts / med / 1e9, 3),
+                                  "frac_of_8tbps": round(byts / med / 1e9 / 8.0, 4)}), flush=True)
+
+
 def batch_specs(n=64, seed=SEED):
     """(L, tau) of the batched-cavities workload: L = round(exp(U[ln 32, ln 160])), tau ~ U[0.51, 1]."""
     rng = np.random.default_rng(seed)
@@ -420,7 +450,7 @@ def batch_study(steps, warmup, group_mb=None):
                       "per_lattice_interleaved_mlups": round(cells * steps / ms_interleaved / 1e3, 1)}), flush=True)
 
 
-def closure_study(nc, tau_fine, chunk, snaps, eval_chunks):
+def closure_study(nc, tau_fine, chunk, snaps, eval_chunks, target="total"):
     """Coarse N^3 with and without the learned closure against the 4x finer run.
 
     The network is trained, and its output scale chosen, on one realisation of
@@ -439,17 +469,29 @@ def closure_study(nc, tau_fine, chunk, snaps, eval_chunks):
     log("# trained on %d samples in %.1f s: loss %.4f -> %.4f" % (t.size, train_s, losses[:20].mean(),
                                                                  losses[-20:].mean()))
     del x, t
-    # output scale (0 = plain LBGK) against the filtered fine run of the training field
-    ke_tr = tr.filtered_ke_history(chunk, snaps)
+    # output scale (0 = plain LBGK) chosen on the training field, over the
+    # evaluation horizon: against the fine run's kinetic-energy decay
+    # (target "total") or the box-filtered fine kinetic energy ("filtered")
+    if target == "total":
+        F = tr.fine()
+        ke_tr, _ = tr.ke_history(F, chunk, eval_chunks, fine=True)
+        del F
+    else:
+        ke_tr = tr.filtered_ke_history(chunk, eval_chunks)
     calib = {}
-    for a in (0.0, 0.25, 0.5, 1.0, 1.5):
+    for a in (0.0, 0.1, 0.2, 0.35, 0.5, 0.75, 1.0, 1.5, 2.0):
         C = tr.coarse()
         if a > 0:
             C.set_closure(params, mu, sc, out * a)
-        ke_a, _ = tr.ke_history(C, chunk, snaps)
-        calib[a] = float(np.abs(ke_a / ke_tr - 1.0).max()) if np.isfinite(ke_a).all() else float("inf")
+        ke_a, _ = tr.ke_history(C, chunk, eval_chunks)
+        if not np.isfinite(ke_a).all():
+            calib[a] = float("inf")
+        elif target == "total":
+            calib[a] = float(np.abs((ke_a / ke_a[0]) / (ke_tr / ke_tr[0]) - 1.0).max())
+        else:
+            calib[a] = float(np.abs(ke_a / ke_tr - 1.0).max())
     alpha = min(calib, key=calib.get)
-    log("# output scale on the training field: %s -> %s" % (calib, alpha))
+    log("# output scale on the training field (%s kinetic energy): %s -> %s" % (target, calib, alpha))
     del tr
     ex = closure.Experiment(n_coarse=nc, tau_fine=tau_fine, seed=SEED)
     F = ex.fine()
@@ -470,7 +512,7 @@ def closure_study(nc, tau_fine, chunk, snaps, eval_chunks):
     def err_filtered(ke):  # against the box-filtered fine run (the field the coarse lattice represents)
         return float(np.abs(ke / ke_ff - 1.0).max())
     print(json.dumps({"study": "learned closure, D3Q19 decaying turbulence", "coarse": nc, "fine": ex.nf,
-                      "tau_fine": tau_fine, "tau_coarse": 0.5 + 3 * ex.nu_c, "coarse_steps_per_chunk": chunk,
+                      "tau_fine": tau_fine, "scale_chosen_against": target, "tau_coarse": 0.5 + 3 * ex.nu_c, "coarse_steps_per_chunk": chunk,
                       "train_snapshots": snaps, "eval_chunks": eval_chunks, "output_scale_factor": alpha,
                       "calibration_errors": {str(k): v for k, v in calib.items()},
                       "ke_fine": ke_f.tolist(), "ke_fine_filtered": ke_ff.tolist(), "ke_coarse_plain": ke_p.tolist(),
@@ -536,6 +578,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline sample")
     ap.add_argument("--remap", type=int, default=-1, metavar="G",
                     help="XCD run length of the step kernel's chunks (default: the library's)")
+    ap.add_argument("--cpl", type=int, default=-1, metavar="C",
+                    help="cells per lane of the step kernel (1, 2, 4; default: the library's choice)")
+    ap.add_argument("--launch-study", action="store_true",
+                    help="step-kernel duration on the sphere channel for every cells-per-lane x XCD run x workgroup")
     ap.add_argument("--autotune", action="store_true",
                     help="let the library's autotuner choose the launch settings before timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -550,6 +596,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=50, help="coarse steps between snapshots")
     ap.add_argument("--train-snapshots", type=int, default=16)
     ap.add_argument("--eval-chunks", type=int, default=16)
+    ap.add_argument("--closure-target", choices=("total", "filtered"), default="total",
+                    help="closure study: choose the output scale against the fine run's kinetic-energy decay "
+                         "or against the box-filtered fine kinetic energy")
     ap.add_argument("--variant-study", type=int, default=0, metavar="N",
                     help="time plain / LBGKTurbulence / closure step kernels on a periodic N^3 lattice")
     ap.add_argument("--cavity-study", action="store_true",
@@ -576,7 +625,8 @@ def main():
         variant_study(args.variant_study)
         return
     if args.closure_study:
-        closure_study(args.closure_study, args.tau_fine, args.chunk, args.train_snapshots, args.eval_chunks)
+        closure_study(args.closure_study, args.tau_fine, args.chunk, args.train_snapshots, args.eval_chunks,
+                      args.closure_target)
         return
     if args.batch_study:
         batch_study(args.steps if args.steps != 5000 else 1000, args.warmup if args.warmup != 200 else 20,
@@ -587,6 +637,9 @@ def main():
         return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.launch_study:
+        launch_study()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this LBGK build runs on one GPU only; --gpus %d is not supported "
@@ -600,10 +653,13 @@ def main():
     from asl_amd import _lib
     # the library's autotuner picks the workgroup size and XCD run length from
     # timed real steps (they advance the flow like the warm-up steps)
+    if args.cpl >= 0:
+        L.set_tuning(_lib.TUNE_CPL, args.cpl)
     block, remap = L.autotune(20) if args.autotune else L.tuning()
     if args.remap >= 0:
         L.set_tuning(_lib.TUNE_XCD_REMAP, args.remap)
         remap = args.remap
+    cpl = L.tuning_value(_lib.TUNE_CPL)
     st = L.stats()
     counted = st["counted"]
     interior = int(np.prod(SIZE))
@@ -654,7 +710,8 @@ def main():
                      "kernel_ms_avg": round(kern_s * 1e3, 5), "kernel_ms_min": round(float(np.min(each)), 5),
                      "kernel_ms_median": round(float(np.median(each)), 5), "kernel_launches_timed": KERNEL_LAUNCHES,
                      "bytes_per_launch": bytes_per_launch, "graph_ms_per_step": round(ms_graph / args.steps, 5),
-                     "launch": {"workgroup": block, "xcd_run": remap}},
+                     "launch": {"workgroup": L.tuning_value(_lib.TUNE_BLOCK), "xcd_run": remap,
+                                "cells_per_lane": cpl}},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -343,13 +343,25 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
  * that lattices over 4 GiB of populations use) whatever the size; 0 = by
  * size (default).  For testing that variant on small lattices. */
 #define ASLHIP_TUNE_WIDE 4
+/* Lattices: cells per lane along z of the step kernel: 1, 2 or 4, or 0 =
+ * automatic (default: one cell per lane).  Runs of 2 or 4 cells apply to
+ * non-periodic lattices under 4 GiB without a per-cell relaxation whose nz
+ * is a multiple of the run (else the longest run that divides nz, down to
+ * one cell); results are bit for bit the same for every value. */
+#define ASLHIP_TUNE_CPL 5
 int aslhip_set_tuning(aslhip_lattice *h, int knob, int value);
-/* The lattice's current workgroup size and XCD run length (either may be NULL). */
+/* The value a tuning knob has now (ASLHIP_TUNE_CPL: the cells per lane the
+ * step kernel runs with). */
+int aslhip_tuning_value(aslhip_lattice *h, int knob, int *value);
+/* The workgroup size the step kernel runs with (runs of 2 or 4 cells per lane:
+ * at most 512) and the XCD run length (either may be NULL). */
 int aslhip_get_tuning(aslhip_lattice *h, int *block, int *remap);
 
 /* The autotuner: times nsteps (rounded up to even) real steps of the lattice
- * for every candidate setting (XCD runs 0/2/4/8/16/64 x workgroup 256/512/1024) with
- * HIP events, keeps the fastest and returns it (either pointer may be NULL).
+ * for every candidate setting (cells per lane 1/2/4 where the lattice can run
+ * them x XCD runs 0/2/4/8/16/64 x workgroup 256/512/1024) with HIP events,
+ * keeps the fastest and returns its workgroup size and XCD run length (either
+ * pointer may be NULL; the cells per lane: aslhip_tuning_value).
  * The trial steps advance the simulation like any other steps. */
 int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_remap);
 

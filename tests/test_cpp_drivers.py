@@ -254,3 +254,185 @@ def test_asl_locomotive_laminar_driver_unchanged(tmp_path):
                      (150.0, 37.0, 53.2), [-10.0, -1.63, -31.6], 0.5, 2.0, float(np.float32(0.01 / 0.5 / 0.5)),
                      False, 200, 1, "locomotive")
 
+
+
+def _surface_box_variant(tri, dx, offs):
+    """readSurface(file, dx, offset_X0, offset_XE, offset_Y0, offset_YE,
+    offset_Z0, offset_ZE) of the compat layer on a triangle set given in file
+    coordinates: the adjusted model bounds (ASL axis order) grown by the
+    offsets, sample counts and spacings, and the lattice's internal block
+    (offset(grid, -1), dx the spacing along ASL's last axis)."""
+    t = tri.astype(np.float32).astype(np.float64)[..., ::-1]
+    pts = t.reshape(-1, 3)
+    lo, hi = pts.min(0), pts.max(0)
+    m = (hi - lo).max()
+    lo, hi = lo - m * 0.0125, hi + m * 0.0125
+    o0, o1 = np.array(offs[0::2]), np.array(offs[1::2])
+    l, u = lo - (hi - lo) * o0, hi + (hi - lo) * o1
+    grid = [int((u[d] - l[d]) / dx) for d in range(3)]
+    h = [(u[d] - l[d]) / (grid[d] - 1) for d in range(3)]
+    return t, grid, list(l), h
+
+
+def _force_nodes(body):
+    """Interior nodes whose surface force ASL defines.  readSurface's dx
+    variants put a body with offset 0 on the block's face (bus_wind and
+    locomotive_stability: Y0), so its shell meets the outer ghost layer: a
+    link from a boundary node (map <= 0) into an outer-layer node with map > 0
+    carries what ASL leaves in memory that no step or BC writes (NaN in the
+    oracle), and those nodes are left out."""
+    import oracle
+    outer = np.ones(body.shape, dtype=bool)
+    outer[(slice(1, -1),) * 3] = False
+    risky = np.zeros(body.shape, dtype=bool)
+    c, _, _ = oracle.template(15)
+    for ci in c[1:]:
+        risky |= np.roll(outer & (body > 0), tuple(-ci), axis=(0, 1, 2))
+    risky &= body <= 0
+    return ~risky[(slice(1, -1),) * 3]
+
+
+def _body_and_block(tri, dx, offs):
+    import oracle
+    t, grid, gpos, h = _surface_box_variant(tri, dx, offs)
+    body = oracle.surface_distance_map(t, grid, gpos, h, 3.6 * dx, 1.8 * dx, dx, 1.8)
+    bdx = h[2]
+    size = tuple(s - 2 for s in grid)
+    ipos = [v + bdx for v in gpos]  # offset(grid, -1)
+    return body, grid, size, ipos, bdx
+
+
+@pytest.mark.gpu
+def test_asl_flow_driver_unchanged(tmp_path):
+    """ASL's examples/flow/flow.cc built unchanged (D3Q15, 300x50x50, pressure
+    drop along x past an infinite cylinder, no-slip walls, the visualisation
+    fill of u): it runs its 1000 steps on the GPU; its one output (written
+    before the loop, after the BCs and the fill) equals the oracle's."""
+    from asl_driver_geometry import cylinder_inf
+    from vti import read_vti
+
+    import oracle
+    exe = os.path.join(BIN, "asl_flow")
+    if not os.path.exists(exe):
+        pytest.skip("ASL's flow drivers were not built (ASL source tree not mounted at build time)")
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=600, cwd=tmp_path)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    out = read_vti(tmp_path / "flow_0.vti")
+    size = (300, 50, 50)
+    S = tuple(s + 2 for s in size)
+    ball = oracle.df_map(cylinder_inf(.125 * 50, [0., 1., 0.], [.45 * 50, .45 * 50, .45 * 50]), S,
+                         (-1.0, -1.0, -1.0), 1.0)
+    inner = (slice(1, -1),) * 3
+    assert np.array_equal(ball[inner], out["map"])
+    O = oracle.OracleLattice(size, 15, fp64=False, nu=float(np.float32(0.00625)))
+    O.init_f(1.0, (0.0, 0.0, 0.0))
+    ids = [O.add_slice_bc(O.CONSTANT_PRESSURE, 1, p=1.2), O.add_slice_bc(O.CONSTANT_PRESSURE, 2, p=0.8),
+           O.add_slice_bc(O.NOSLIP, 4 | 8 | 16 | 32), O.add_map_bc(O.NOSLIP_MAP, ball)]
+    for b in ids:
+        O.execute_bc(b)
+    O.vis_fill(1, ball)
+    guard = (ball[inner] <= 0) & (ball[inner] > -0.9999)
+    keep = (ball[inner] > 0) | guard
+    assert guard.sum() > 1000
+    assert np.array_equal(out["rho"][ball[inner] > 0], O.rho()[inner].astype(np.float32)[ball[inner] > 0])
+    assert np.array_equal(out["v"][:, keep], O.velocity()[(slice(None),) + inner].astype(np.float32)[:, keep])
+
+
+@pytest.mark.gpu
+def test_asl_bus_wind_driver_unchanged(tmp_path):
+    """ASL's examples/flow/bus_wind.cc built unchanged on a box-shaped STL
+    body (readSurface(file, dx 8, offsets), D3Q15, pressure-velocity on all six
+    faces, the visualisation fill and the surface force): it runs its 100
+    steps on the GPU; its one output (written before the loop) equals the
+    oracle's, body map, rho, u (fill included) and force bit for bit."""
+    from test_oracle_surface_map import cube_triangles, write_stl_binary
+    from vti import read_vti
+
+    import oracle
+    exe = os.path.join(BIN, "asl_bus_wind")
+    if not os.path.exists(exe):
+        pytest.skip("ASL's flow drivers were not built (ASL source tree not mounted at build time)")
+    tri = cube_triangles(np.array([0.0, 0.0, 0.0]), np.array([400.0, 100.0, 120.0]))
+    write_stl_binary(tmp_path / "bus.stl", tri)
+    p = subprocess.run([exe, "--input", str(tmp_path / "bus.stl")], capture_output=True, text=True, timeout=600,
+                       cwd=tmp_path)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    out = read_vti(tmp_path / "bus_wind_0.vti")
+    body, grid, size, _, _ = _body_and_block(tri, 8.0, (1.5, .25, 0., 1., 3., 1.))
+    inner = (slice(1, -1),) * 3
+    assert np.array_equal(body[inner], out["bus"])
+    U = (-0.1, 0.0, -0.05)
+    O = oracle.OracleLattice(size, 15, fp64=False, nu=float(np.float32(0.01)))
+    O.init_f(1.0, U)
+    ids = [O.add_map_bc(O.NOSLIP_MAP, body), O.add_slice_bc(O.CONSTANT_PRESSURE_VELOCITY, 63, p=1.0, u=U)]
+    for b in ids:
+        O.execute_bc(b)
+    O.vis_fill(1, body)
+    guard = (body[inner] <= 0) & (body[inner] > -0.9999)
+    keep = (body[inner] > 0) | guard
+    assert guard.sum() > 1000
+    assert np.array_equal(out["rho"][body[inner] > 0], O.rho()[inner].astype(np.float32)[body[inner] > 0])
+    assert np.array_equal(out["v"][:, keep], O.velocity()[(slice(None),) + inner].astype(np.float32)[:, keep])
+    # the mirrored stream (0 where a boundary node has no fluid neighbour)
+    assert (out["v"][0][guard] >= 0).all() and (out["v"][0][guard] > 0).mean() > 0.5
+    ok = _force_nodes(body)
+    fo = O.surface_force(body)[(slice(None),) + inner].astype(np.float32)
+    assert ok.sum() > 0.99 * ok.size and np.abs(fo[:, ok]).max() > 1e-4
+    assert np.array_equal(out["force"][:, ok], fo[:, ok])
+
+
+@pytest.mark.gpu
+def test_asl_locomotive_stability_driver_unchanged(tmp_path):
+    """ASL's examples/flow/locomotive_stability.cc built unchanged on a
+    box-shaped STL body placed under the driver's four nozzles
+    (readSurface(file, dx 0.25, offsets), D3Q15, pressure-velocity on all six
+    faces and on the nozzle map, the visualisation fill, the surface force):
+    it runs its 40,000 steps on the GPU; its output after 1000 steps equals
+    the oracle bit for bit (maps, rho and u on the fluid nodes, force)."""
+    from asl_driver_geometry import cylinder
+    from test_oracle_surface_map import cube_triangles, write_stl_binary
+    from vti import read_vti
+
+    from asl_amd import geometry as g
+    import oracle
+    exe = os.path.join(BIN, "asl_locomotive_stability")
+    if not os.path.exists(exe):
+        pytest.skip("ASL's flow drivers were not built (ASL source tree not mounted at build time)")
+    # file (x, y, z) = ASL (z, y, x): the body spans ASL x in [-16, -6], y in [7, 9], z in [3, 4]
+    tri = cube_triangles(np.array([3.0, 7.0, -16.0]), np.array([4.0, 9.0, -6.0]))
+    write_stl_binary(tmp_path / "body.stl", tri)
+    p = subprocess.run([exe, "--input", str(tmp_path / "body.stl")], capture_output=True, text=True,
+                       timeout=900, cwd=tmp_path)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    out = read_vti(tmp_path / "locomotive_stability_1.vti")
+    dx = float(np.float32(0.25))  # asl::Parameter<float> dx, widened to double
+    body, grid, size, ipos, bdx = _body_and_block(tri, dx, (.5, 1., 0., 1., 2., 4.))
+    inner = (slice(1, -1),) * 3
+    assert np.array_equal(body[inner], out["locomotive"])
+    # generateNozzels(block): four cylinders r 0.3, height 1 along z, normalised by the block's dx
+    noz = None
+    for i in range(4):
+        c = cylinder(.3, [0., 0., 1.], [-20. + i * (-6. + 20.) / 4., 10.33, 3.08 + .5])
+        noz = c if noz is None else g.df_or(noz, c)
+    tpos = [v - bdx for v in ipos]
+    nozzles = oracle.df_map(g.df_normalize(noz, bdx), grid, tpos, bdx)
+    assert np.array_equal(nozzles[inner], out["nozzels"])
+    assert (nozzles[inner] <= 0).sum() > 20
+    O = oracle.OracleLattice(size, 15, fp64=False, nu=float(np.float32(0.01)))
+    O.init_f(1.0, (0.1, 0.0, 0.05))
+    ids = [O.add_map_bc(O.NOSLIP_MAP, body),
+           O.add_slice_bc(O.CONSTANT_PRESSURE_VELOCITY, 63, p=1.0, u=(0.1, 0.0, 0.05)),
+           O.add_map_bc(O.PRESSURE_VELOCITY_MAP, nozzles, p=1.0, u=(0.0, 0.0, -0.1))]
+    for b in ids:
+        O.execute_bc(b)
+    O.run(1000, ids)
+    fluid = (body[inner] > 0) & (nozzles[inner] > 0)
+    assert fluid.sum() > 20000
+    v_o = O.velocity()[(slice(None),) + inner].astype(np.float32)
+    assert np.abs(v_o[:, fluid]).max() > 0.05
+    assert np.array_equal(out["rho"][fluid], O.rho()[inner].astype(np.float32)[fluid])
+    assert np.array_equal(out["v"][:, fluid], v_o[:, fluid])
+    ok = _force_nodes(body)
+    fo = O.surface_force(body)[(slice(None),) + inner].astype(np.float32)
+    assert ok.sum() > 0.99 * ok.size and np.abs(fo[:, ok]).max() > 1e-4
+    assert np.array_equal(out["force"][:, ok], fo[:, ok])

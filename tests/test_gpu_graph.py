@@ -94,13 +94,17 @@ def test_launch_tuning_and_autotune_keep_results():
         L.step(70, graph=True)
         L.step(50)
         same(L, ref)
-    ref.step(180)
     L, _ = cavity()
     assert L.tuning() == (512, 4)  # the defaults
-    best = L.autotune(10)  # 18 candidates x 10 real steps
+    # candidates: cells per lane 1 and 2 (nz = 22 is not a multiple of 4) x 6 XCD
+    # runs x 3 workgroup sizes, 10 real steps each
+    ncand = 2 * 6 * 3
+    ref.step(10 * ncand)  # ref: 120 + 10 ncand steps; L: 10 ncand + 120
+    best = L.autotune(10)
     assert best[0] in (256, 512, 1024) and best[1] in (0, 2, 4, 8, 16, 64)
+    assert L.tuning_value(_lib.TUNE_CPL) in (1, 2)
     assert L.tuning() == best
-    L.step(300 - 180)
+    L.step(120)
     same(L, ref)
     with pytest.raises(asl_amd.ASLError):
         L.set_tuning(_lib.TUNE_BLOCK, 100)
