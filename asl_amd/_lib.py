@@ -28,6 +28,7 @@ DF_SPHERE, DF_CYLINDER, DF_CONE, DF_PLANE, DF_AND, DF_OR, DF_NEG, DF_NORMALIZE =
 CLOSURE_NPARAM = 1345
 CLOSURE_FP32, CLOSURE_BF16_MFMA = 0, 1
 VIS_NOSLIP_VEL, VIS_NOSLIP_RHO, VIS_VELOCITY = 1, 2, 3
+NOALLOC_SCRATCH_BYTES = 16 << 20
 
 # Every entry point of include/aslhip.h with its ctypes signature.
 _vp = ctypes.c_void_p
@@ -58,6 +59,7 @@ SIGNATURES = {
     "aslhip_read_rho": [_vp, _vp],
     "aslhip_read_velocity": [_vp, _vp],
     "aslhip_read_f": [_vp, _vp],
+    "aslhip_read_f_noalloc": [_vp, _vp],
     "aslhip_write_f": [_vp, _vp],
     "aslhip_surface_force": [_vp, _fp, _vp],
     "aslhip_surface_flux": [_vp, _fp, _vp],

@@ -26,6 +26,8 @@
 // operation rounds once, as in the CPU oracle.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "../../include/aslhip.h"
 #include "kernels.hpp"
 
@@ -1324,6 +1326,33 @@ struct ReduceF {
 };
 
 }  // namespace
+
+// Staging memory of aslhip_read_f_noalloc: a file-scope device array, so the
+// entry point needs no allocation.  In host code `scratch` only names the
+// symbol; its device address on each GPU comes from hipGetSymbolAddress (the
+// array's name is never passed to a kernel or a copy).
+static __device__ __align__(256) unsigned char scratch[ASLHIP_NOALLOC_SCRATCH_BYTES];
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[SCRATCH_MAX_DEVICES] = {};  // its address on each device (null: not looked up yet)
+}  // namespace
+
+hipError_t noalloc_scratch(void **ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    scratch_ptr[dev] = p;
+  }
+  *ptr = scratch_ptr[dev];
+  return hipSuccess;
+}
 
 int reduce_blocks(int len) {
   const int nb = (len + BLOCK - 1) / BLOCK;

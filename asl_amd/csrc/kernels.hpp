@@ -201,6 +201,10 @@ hipError_t launch_peek(int q, int dtype, bool odd, bool periodic, const StepLaun
 hipError_t launch_surface(int q, int dtype, const SurfaceArgs &a, hipStream_t s);
 hipError_t launch_vis(int q, int dtype, const VisArgs &a, hipStream_t s);
 hipError_t launch_unpermute(int q, int dtype, const UnpermuteArgs &a, hipStream_t s);
+// The allocation-free staging array (ASLHIP_NOALLOC_SCRATCH_BYTES bytes of
+// device memory in the library's code object, one per device): its address
+// on the current device, looked up once per device.
+hipError_t noalloc_scratch(void **ptr);
 // partial must hold 5 * reduce_blocks(len) doubles; out: 5 doubles (device).
 int reduce_blocks(int len);
 hipError_t launch_reduce(int q, int dtype, const ReduceArgs &a, double *out, hipStream_t s);

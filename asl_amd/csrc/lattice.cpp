@@ -118,6 +118,7 @@ struct aslhip_lattice {
   size_t selem = 4, melem = 4, celem = 4;  // storage / macroscopic / compute element bytes
   long long qs_bytes = 0;
   hipStream_t stream = nullptr;
+  int device = -1;  // the device current at creation (its memory and stream live there)
   // device buffers
   char *f = nullptr;
   void *rho = nullptr, *u = nullptr;
@@ -955,7 +956,8 @@ int aslhip_lattice_create(aslhip_lattice **out, int nx, int ny, int nz, int q, i
   if (qs_elems * (dtype == ASLHIP_F64 ? 8 : 4) > 0xffffffffLL)
     return fail(ASLHIP_EINVAL, "ASL ERROR: block too large: one direction array must stay below 4 GiB");
   h->qs_bytes = qs_elems * (long long)h->selem;
-  hipError_t e = hipStreamCreate(&h->stream);
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipStreamCreate(&h->stream);
   if (e == hipSuccess) e = hipMalloc(&h->f, (size_t)h->qs_bytes * q);
   if (e == hipSuccess) e = hipMalloc(&h->rho, (size_t)len * h->melem);
   if (e == hipSuccess) e = hipMalloc(&h->u, (size_t)len * 3 * h->melem);
@@ -1349,6 +1351,27 @@ int aslhip_macro_device(aslhip_lattice *h, void *stream, const void **rho, const
   if (rho) *rho = h->rho;
   if (u) *u = h->u;
   return 0;
+}
+
+int aslhip_read_f_noalloc(aslhip_lattice *h, void *out) {
+  if (int r = check(h)) return r;
+  if (!out) return fail(ASLHIP_EINVAL, "ASL ERROR: read_f_noalloc needs an output buffer");
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != h->device)
+    return fail(ASLHIP_EINVAL, "ASL ERROR: read_f_noalloc: the lattice lives on device %d, device %d is current",
+                h->device, dev);
+  if ((size_t)h->Q * h->melem > ASLHIP_NOALLOC_SCRATCH_BYTES)
+    return fail(ASLHIP_EINVAL, "ASL ERROR: read_f_noalloc: one cell's populations exceed the %u-byte staging array",
+                ASLHIP_NOALLOC_SCRATCH_BYTES);
+  if (h->deferred && (h->dirty || !h->pending.empty()))
+    return fail(ASLHIP_ESTATE, "ASL ERROR: read_f_noalloc: the pending step needs its boundary tables rebuilt "
+                               "(an allocation); call aslhip_finalize first");
+  void *buf = nullptr;
+  HIPCHK(noalloc_scratch(&buf));
+  int r = aslhip::read_f_through(h, out, buf, ASLHIP_NOALLOC_SCRATCH_BYTES);
+  (void)hipStreamSynchronize(h->stream);
+  return r;
 }
 
 int aslhip_read_f(aslhip_lattice *h, void *out) {

@@ -314,6 +314,23 @@ int aslhip_read_velocity(aslhip_lattice *h, void *out);
 int aslhip_macro_device(aslhip_lattice *h, void *stream, const void **rho, const void **u);
 int aslhip_read_f(aslhip_lattice *h, void *out);
 
+/* No counterpart in ASL.  aslhip_read_f for callers that cannot allocate
+ * device memory (tight host loops that checkpoint, harnesses that forbid
+ * hipMalloc): the view is un-permuted through a fixed staging array of
+ * ASLHIP_NOALLOC_SCRATCH_BYTES in the library's device code (one per GPU,
+ * held until the process ends) instead of a buffer allocated per call, in
+ * pieces of at most that size, so any lattice fits; the only limit, one
+ * cell's Q values (216 bytes at most), is checked.  Same output as
+ * aslhip_read_f.  The lattice's device must be the current one.  A pending
+ * step runs first, as for aslhip_read_f; one whose boundary tables must be
+ * rebuilt (BCs registered since the last step) is refused, since that
+ * set-up allocates: call aslhip_finalize first.  The staging array keeps the
+ * previous call's contents; calls that overlap on one GPU (from several host
+ * threads) race on it.  It synchronises the host with the lattice's stream,
+ * so it does not belong under graph capture. */
+#define ASLHIP_NOALLOC_SCRATCH_BYTES (16u << 20)
+int aslhip_read_f_noalloc(aslhip_lattice *h, void *out);
+
 /* Packed cell-type map (one byte per ghost-inclusive cell):
  * 0 fluid, 1 fluid with BC links, 2 not computed (ghost/solid),
  * 3 computed ghost.  Bits 4-7 unused. */
