@@ -24,6 +24,7 @@ TUNE_BLOCK = 2
 TUNE_GROUP_MB = 3
 TUNE_WIDE = 4
 TUNE_CPL = 5
+TUNE_LDS = 6
 DF_SPHERE, DF_CYLINDER, DF_CONE, DF_PLANE, DF_AND, DF_OR, DF_NEG, DF_NORMALIZE = 1, 2, 3, 4, 5, 6, 7, 8
 CLOSURE_NPARAM = 1345
 CLOSURE_FP32, CLOSURE_BF16_MFMA = 0, 1

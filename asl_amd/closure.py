@@ -262,5 +262,91 @@ def train(x, t, params0, steps=3000, batch=8192, lr=2e-3, seed=1):
     return m.params(), mu, sc, out, losses
 
 
+# output scales tried on the training field (0: plain LBGK)
+SCALE_GRID = (0.0, 0.1, 0.15, 0.2, 0.25, 0.3, 0.35, 0.5, 0.75, 1.0, 1.5, 2.0)
+
+
+def _ke_decay_error(ke, ke_ref):
+    """Largest relative deviation of the kinetic-energy decay ke / ke[0] from
+    the reference's (inf if the run blew up)."""
+    if not np.isfinite(ke).all():
+        return float("inf")
+    return float(np.abs((ke / ke[0]) / (ke_ref / ke_ref[0]) - 1.0).max())
+
+
+def _ke_error(ke, ke_ref):
+    if not np.isfinite(ke).all():
+        return float("inf")
+    return float(np.abs(ke / ke_ref - 1.0).max())
+
+
+def study(n_coarse, tau_fine, chunk=50, snapshots=16, eval_chunks=16, target="total", seed=20261202,
+          train_steps=4000, batch=16384, grid=SCALE_GRID, log=print):
+    """Coarse n^3 with and without the learned closure against the 4x finer run.
+
+    The network is trained, and its output scale chosen from `grid`, on one
+    realisation of the initial field (seed + 1): against the fine run's
+    kinetic-energy decay (target "total") or the box-filtered fine kinetic
+    energy ("filtered"), over the evaluation horizon of eval_chunks chunks of
+    `chunk` coarse steps.  The comparison runs on another realisation (seed).
+    Returns a dict of the histories, errors and wall times.
+    """
+    t0 = time.perf_counter()
+    tr = Experiment(n_coarse=n_coarse, tau_fine=tau_fine, seed=seed + 1)
+    log("# closure study: coarse %d^3, fine %d^3, tau_fine %.4f; training field ready (%.1f s)"
+        % (n_coarse, tr.nf, tau_fine, time.perf_counter() - t0))
+    x, t = tr.training_data(chunk, snapshots, log=log)
+    tt = time.perf_counter()
+    params, mu, sc, out, losses = train(x, t, init_params(1), steps=train_steps, batch=batch, lr=2e-3)
+    train_s = time.perf_counter() - tt
+    log("# trained on %d samples in %.1f s: loss %.4f -> %.4f" % (t.size, train_s, losses[:20].mean(),
+                                                                 losses[-20:].mean()))
+    del x, t
+    if target == "total":
+        F = tr.fine()
+        ke_tr, _ = tr.ke_history(F, chunk, eval_chunks, fine=True)
+        del F
+    else:
+        ke_tr = tr.filtered_ke_history(chunk, eval_chunks)
+    calib = {}
+    for a in grid:
+        C = tr.coarse()
+        if a > 0:
+            C.set_closure(params, mu, sc, out * a)
+        ke_a, _ = tr.ke_history(C, chunk, eval_chunks)
+        calib[a] = _ke_decay_error(ke_a, ke_tr) if target == "total" else _ke_error(ke_a, ke_tr)
+        del C
+    alpha = min(calib, key=calib.get)
+    log("# output scale on the training field (%s kinetic energy): %s -> %s" % (target, calib, alpha))
+    del tr
+    ex = Experiment(n_coarse=n_coarse, tau_fine=tau_fine, seed=seed)
+    F = ex.fine()
+    ke_f, wall_f = ex.ke_history(F, chunk, eval_chunks, fine=True)
+    del F
+    log("# fine run done (%.1f s)" % wall_f)
+    ke_ff = ex.filtered_ke_history(chunk, eval_chunks)
+    log("# filtered fine kinetic energy done")
+    P = ex.coarse()
+    ke_p, wall_p = ex.ke_history(P, chunk, eval_chunks)
+    del P
+    C = ex.coarse()
+    C.set_closure(params, mu, sc, out * max(alpha, 1e-30))
+    ke_c, wall_c = ex.ke_history(C, chunk, eval_chunks)
+    del C
+    return {"study": "learned closure, D3Q19 decaying turbulence", "coarse": n_coarse, "fine": ex.nf,
+            "tau_fine": tau_fine, "scale_chosen_against": target, "tau_coarse": 0.5 + 3 * ex.nu_c,
+            "coarse_steps_per_chunk": chunk, "train_snapshots": snapshots, "eval_chunks": eval_chunks,
+            "output_scale_factor": alpha, "calibration_errors": {str(k): v for k, v in calib.items()},
+            "ke_fine": ke_f.tolist(), "ke_fine_filtered": ke_ff.tolist(), "ke_coarse_plain": ke_p.tolist(),
+            "ke_coarse_closure": ke_c.tolist(),
+            "max_rel_ke_decay_error_plain": _ke_decay_error(ke_p, ke_f),
+            "max_rel_ke_decay_error_closure": _ke_decay_error(ke_c, ke_f),
+            "max_rel_ke_error_vs_filtered_plain": _ke_error(ke_p, ke_ff),
+            "max_rel_ke_error_vs_filtered_closure": _ke_error(ke_c, ke_ff),
+            "wall_s_fine": wall_f, "wall_s_coarse_plain": wall_p, "wall_s_coarse_closure": wall_c,
+            "speedup_closure_vs_fine": wall_f / wall_c, "train_s": train_s,
+            "train_loss_first": float(losses[:20].mean()), "train_loss_last": float(losses[-20:].mean())}
+
+
 __all__ = ["Experiment", "turbulence_field", "box_filter", "eddy_viscosity", "strain_rate", "train", "init_params",
-           "normalisation", "with_ghosts"]
+           "normalisation", "with_ghosts", "study", "SCALE_GRID"]

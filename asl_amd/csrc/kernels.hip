@@ -703,6 +703,271 @@ __device__ __forceinline__ void step_body_run(const StepArgs &a, const int L) {
   }
 }
 
+// Direction components packed two bits each ((c + 1) << 2k), so that the
+// offset of a direction chosen at run time is a few scalar instructions.
+template <int Q>
+constexpr unsigned long long packed_dir(int d) {
+  unsigned long long m = 0;
+  for (int k = 0; k < Q; ++k) m |= static_cast<unsigned long long>(Lat<Q>::c[k][d] + 1) << (2 * k);
+  return m;
+}
+
+// Offset of c_j in the layout for a direction j known at run time.
+template <int Q>
+__device__ __forceinline__ int offc_rt(int j, int sx, int sy) {
+  constexpr unsigned long long PX = packed_dir<Q>(0), PY = packed_dir<Q>(1), PZ = packed_dir<Q>(2);
+  const int cx = static_cast<int>((PX >> (2 * j)) & 3ull) - 1;
+  const int cy = static_cast<int>((PY >> (2 * j)) & 3ull) - 1;
+  const int cz = static_cast<int>((PZ >> (2 * j)) & 3ull) - 1;
+  return cx * sx + cy * sy + cz;
+}
+
+// LDS-staged step: a workgroup of STEP_TILE lanes takes STEP_TILE consecutive
+// cells of a plane (chunks as in step_body, ghost columns included; the last
+// chunk of a plane is shorter).  In the AA pattern the slots the tile's cells
+// use in direction array j form one contiguous range ("row" j): cells
+// idx0 .. idx0 + n - 1 on even steps, shifted by the offset of c_j on odd
+// steps.  So the kernel
+//   1. loads every row into LDS with 16-byte loads,
+//   2. lets each lane collide its own cell from LDS (one cell per lane, the
+//      same link resolution and collision as step_body) and write its
+//      outgoing populations into the rows in place, in its own column,
+//   3. stores the rows back to the ranges they came from with 16-byte stores.
+// Column t of the rows holds exactly the slots cell idx0 + t reads and
+// writes: incoming k is row k (even) or row inv k (odd), outgoing k goes to
+// row inv k (even) or row k (odd).  Each slot belongs to exactly one cell,
+// so tiles never touch each other's slots, and cells that are not computed
+// keep their slots as loaded.  A bounce-back link k reads the other slot of
+// its pair (even: array inv k at x - c_k; odd: array k at x), which is the
+// outgoing row of k at column t - (offset of c_k): from LDS if that column
+// lies in the tile, else from global memory (read before the first barrier
+// for face cells, after it for link-pattern cells; the slot belongs to a
+// cell that is not computed, so its value does not change during the step).
+template <int Q, class S, bool ODD>
+__device__ __forceinline__ void step_body_tile(const StepArgs &a, const int L) {
+  using CD = Codec<S>;
+  using C = typename CD::C;
+  using M = typename CD::M;
+  constexpr int TB = STEP_TILE;
+  constexpr int NW = TB / 64;             // wavefronts per workgroup
+  constexpr int RPW = (Q + NW - 1) / NW;  // rows per wavefront, at most
+  constexpr int EPL = TB / 64;            // row elements per lane
+  constexpr unsigned ES = sizeof(S);
+  static_assert(TB % 64 == 0 && (EPL & (EPL - 1)) == 0, "tile: a power-of-two multiple of 64 cells");
+  typedef S V4 __attribute__((ext_vector_type(EPL), aligned(sizeof(S))));  // global: element alignment
+  typedef S L4 __attribute__((ext_vector_type(EPL)));                       // LDS: EPL elements aligned
+  __shared__ __attribute__((aligned(16 * sizeof(S)))) S row[Q][TB];
+  const int t = static_cast<int>(threadIdx.x);
+  const int plane = L / a.bpp;
+  const int chunk = L - plane * a.bpp;
+  const int c0 = chunk * TB;                // first cell of the tile in the plane's rows
+  const int n = min(TB, a.ny * a.sy - c0);  // cells in the tile
+  const long long idx0 = static_cast<long long>(plane + 1) * a.sx + a.sy + c0;
+  const long long len = a.len;
+  const char *f0 = a.fd[0];
+  // first cell of row j's range (j at run time: scalar arithmetic)
+  auto row_base = [&](int j) -> long long { return ODD ? idx0 + offc_rt<Q>(j, a.sx, a.sy) : idx0; };
+  auto in_row = [](int k) -> int { return ODD ? inv_dir<Q>(k) : k; };   // incoming k, column t
+  auto out_row = [](int k) -> int { return ODD ? k : inv_dir<Q>(k); };  // outgoing k, column t
+
+  // this lane's cell
+  const bool in_tile = t < n;
+  const unsigned off = static_cast<unsigned>(c0 + (in_tile ? t : 0));
+  const unsigned rw = static_cast<unsigned>((static_cast<unsigned long long>(off) * a.zmagic) >> 40);
+  const int zi = static_cast<int>(off - rw * static_cast<unsigned>(a.sy));
+  const int x = plane + 1, y = static_cast<int>(rw) + 1;
+  const unsigned idx = static_cast<unsigned>(idx0 + (in_tile ? t : 0));
+  const bool column = in_tile && zi != 0 && zi != a.sy - 1;  // not a ghost column
+  const unsigned char code = column ? *gload<unsigned char>(reinterpret_cast<const char *>(a.code) + idx) : CODE_SKIP;
+
+  // 1. rows into LDS: wavefront w moves rows w, w + NW, w + 2 NW, ..., lane
+  // t elements e .. e + EPL - 1 (e = EPL (t % 64)) of each; every load is issued
+  // before the first wait.  A vector whose range leaves the lattice's cells
+  // [0, len) (only at the two ends of the lattice) is loaded from cell 0 and
+  // then replaced element by element; elements past the tile's n cells are
+  // loaded but never used.
+  const int e = EPL * (t & 63);
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  {
+    V4 xr[RPW];
+    bool vok[RPW];
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const int j = w + i * NW;
+      const long long g = j < Q ? row_base(j) + e : 0;
+      vok[i] = g >= 0 && g + EPL <= len;
+      const unsigned gi = vok[i] ? static_cast<unsigned>(g) : 0u;
+      if (j < Q) xr[i] = *gload<V4>(f0 + static_cast<unsigned>(j) * a.qs32 + gi * ES);
+    }
+    bool fix = false;
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) fix = fix || (w + i * NW < Q && !vok[i]);
+    if (fix) {
+#pragma unroll
+      for (int i = 0; i < RPW; ++i) {
+        const int j = w + i * NW;
+        if (j >= Q || vok[i]) continue;
+        const long long g = row_base(j) + e;
+#pragma unroll
+        for (int q = 0; q < EPL; ++q)
+          if (g + q >= 0 && g + q < len)
+            xr[i][q] = *gload<S>(f0 + static_cast<unsigned>(j) * a.qs32 + static_cast<unsigned>(g + q) * ES);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const int j = w + i * NW;
+      if (j < Q) {
+        L4 v4;
+        __builtin_memcpy(&v4, &xr[i], sizeof v4);
+        *reinterpret_cast<L4 *>(&row[j][e]) = v4;
+      }
+    }
+  }
+
+  // face rule (as step_body)
+  const bool raw = (a.flags & STEP_F_RAW) != 0;
+  const unsigned fb = (!column || raw) ? 0u
+                                       : (x == 1 ? 1u : 0u) | (x == a.nx ? 2u : 0u) | (y == 1 ? 4u : 0u) |
+                                             (y == a.ny ? 8u : 0u) | (zi == 1 ? 16u : 0u) | (zi == a.nz ? 32u : 0u);
+  auto winner = [&](int k, C &val) -> int {
+    int wn = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int cd = Lat<Q>::c[k][d];
+      if (cd == 0) continue;
+      const int fc = 2 * d + (cd > 0 ? 0 : 1);
+      const int cand = ((fb >> fc) & 1u) ? a.fface[fc] : 0;
+      if (cand > wn) {
+        wn = cand;
+        val = face_val<C>(a, fc, k);
+      }
+    }
+    return wn;
+  };
+  const bool faces = __any(fb != 0u);
+  unsigned bmask = 0, cmask = 0;
+  if (faces) {
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+      C unused;
+      const int md = winner(k, unused) & 3;
+      if (md == MODE_BOUNCE) bmask |= 1u << k;
+      if (md == MODE_CONST) cmask |= 1u << k;
+    }
+  }
+  // the other slot of link k's pair: the outgoing row of k at column t - (offset of c_k)
+  auto other_col = [&](int k) -> int { return t - offc<Q>(k, a.sx, a.sy); };
+  auto other_in_tile = [&](int k) -> bool {
+    const int oc = other_col(k);
+    return oc >= 0 && oc < n;
+  };
+  auto other_global = [&](int k) -> C {
+    const int j = out_row(k);
+    const long long g = (ODD ? idx0 + offc<Q>(j, a.sx, a.sy) : idx0) + other_col(k);
+    return CD::ld(f0, static_cast<unsigned>(j) * a.qs32 + static_cast<unsigned>(g) * ES);
+  };
+  C side[Q];  // bounce-back values of face cells from outside the tile
+  if (faces) {
+#pragma unroll
+    for (int k = 1; k < Q; ++k)
+      if (((bmask >> k) & 1u) && !other_in_tile(k)) side[k] = other_global(k);
+  }
+  __syncthreads();
+
+  // 2. collide this lane's cell from the rows
+  const int tc = in_tile ? t : 0;
+  C v[Q];
+#pragma unroll
+  for (int k = 0; k < Q; ++k) v[k] = CD::dec(row[in_row(k)][tc]);
+  if (faces) {
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+      if (!(((bmask | cmask) >> k) & 1u)) continue;
+      C val = C(0);
+      (void)winner(k, val);
+      if ((bmask >> k) & 1u) {
+        const C o = other_in_tile(k) ? CD::dec(row[out_row(k)][other_col(k)]) : side[k];
+        v[k] = o + val;
+      } else {
+        v[k] = val;
+      }
+    }
+  }
+  const bool live = code != CODE_SKIP;
+  const bool bnd = live && code != CODE_FLUID;
+  const bool general = __any(bnd);
+  unsigned skipm = 0;
+  if (general && bnd) {
+    const LinkRec rec = a.rec[*gload<unsigned short>(reinterpret_cast<const char *>(a.pat) + 2u * idx)];
+    skipm = rec.skip;
+    const C *val = static_cast<const C *>(a.val);
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+      const int slot = raw ? 0 : rec.slot[k];
+      const unsigned char md = slot ? a.mode[slot] : static_cast<unsigned char>(MODE_NONE);
+      C xk = CD::dec(row[in_row(k)][t]);
+      if (md == MODE_BOUNCE)
+        xk = other_in_tile(k) ? CD::dec(row[out_row(k)][other_col(k)]) : other_global(k);
+      const C sv = val[slot * Q + k];
+      v[k] = md == MODE_BOUNCE ? xk + sv : (md == MODE_CONST ? sv : xk);
+    }
+  }
+  // every lane has read what it needs from the rows before any lane writes them
+  __syncthreads();
+  C r, m[3];
+  const bool omega = (a.flags & STEP_F_OMEGA) != 0;
+  const bool comp = (a.flags & STEP_F_COMPRESSIBLE) != 0;
+  C om[3] = {C(0), C(0), C(0)};
+  if (omega) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) om[d] = sizeof(C) == 4 ? C(a.om32[d]) : C(a.om64[d]);
+  }
+  collide<Q, C>(v, C(a.w), comp, omega, om, r, m);
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < Q; ++k)
+      if (!((skipm >> k) & 1u)) row[out_row(k)][t] = CD::enc(v[k]);
+    if (a.flags & STEP_F_RHO) *gstore<M>(static_cast<char *>(a.rho) + sizeof(M) * idx) = M(r);
+    if (a.flags & STEP_F_VEL) {
+      char *u = static_cast<char *>(a.u);
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        *gstore<M>(u + sizeof(M) * (static_cast<size_t>(d) * a.len + idx)) = M(comp ? m[d] / r : m[d]);
+    }
+  }
+  __syncthreads();
+
+  // 3. rows back to their ranges
+  {
+    char *fw = a.fd[0];
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const int j = w + i * NW;
+      if (j >= Q || e >= n) continue;
+      const long long g = row_base(j) + e;
+      if (e + EPL <= n && g >= 0 && g + EPL <= len) {
+        const L4 l4 = *reinterpret_cast<const L4 *>(&row[j][e]);
+        V4 x4;
+        __builtin_memcpy(&x4, &l4, sizeof x4);
+        *gstore<V4>(fw + static_cast<unsigned>(j) * a.qs32 + static_cast<unsigned>(g) * ES) = x4;
+      } else {
+#pragma unroll
+        for (int q = 0; q < EPL; ++q)
+          if (e + q < n && g + q >= 0 && g + q < len)
+            *gstore<S>(fw + static_cast<unsigned>(j) * a.qs32 + static_cast<unsigned>(g + q) * ES) = row[j][e + q];
+      }
+    }
+  }
+}
+
+template <int Q, class S, bool ODD>
+__global__ __launch_bounds__(STEP_TILE) void k_step_tile(const StepArgs *args) {
+  const StepArgs &a = *args;
+  const int b = ODD ? xcd_reverse(static_cast<int>(blockIdx.x), a.nblocks) : static_cast<int>(blockIdx.x);
+  step_body_tile<Q, S, ODD>(a, a.remap > 1 ? xcd_remap(b, a.nblocks, a.remap) : b);
+}
+
 // The arguments live in device memory, not in the kernel-argument segment:
 // loads from the argument segment are invariant, and the compiler hoists all
 // of them (27 direction pointers, face tables) into scalar registers, runs
@@ -1169,6 +1434,12 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_final(const double *partial, i
 template <int Q, class S, bool ODD, bool PER>
 hipError_t step_w(bool wide, const StepLaunch &a, hipStream_t s) {
   const dim3 grid(a.nblocks), block(a.block);
+  if (a.tile) {
+    // the host sends only these: non-periodic, under 4 GiB, no per-cell relaxation, one cell per lane
+    if (PER || wide || a.variant || a.cpl != 1 || a.block != STEP_TILE) return hipErrorInvalidValue;
+    if constexpr (!PER) hipLaunchKernelGGL((k_step_tile<Q, S, ODD>), grid, block, 0, s, a.args);
+    return hipGetLastError();
+  }
   if (a.cpl > 1) {
     if (PER || wide || a.variant) return hipErrorInvalidValue;  // the host sends these one cell per lane
     if (a.cpl == 2)

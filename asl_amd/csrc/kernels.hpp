@@ -8,9 +8,11 @@
 namespace aslhip {
 
 // Threads per workgroup of the step kernel (each thread one cell), at most;
-// of its variant with runs of 2 or 4 cells per lane, at most.
+// of its variant with runs of 2 or 4 cells per lane, at most; of its
+// LDS-staged variant (one tile of STEP_TILE cells per workgroup), exactly.
 constexpr int STEP_BLOCK = 1024;
 constexpr int STEP_RUN_BLOCK = 512;
+constexpr int STEP_TILE = 256;
 
 // Step-kernel flags.
 enum : unsigned {
@@ -53,6 +55,7 @@ struct StepArgs {
   int cpl;                    // cells per lane along z (1: one cell per lane, ghost columns included)
   int nzv;                    // runs per row, nz / cpl (cpl > 1)
   unsigned long long vmagic;  // ceil(2^40 / nzv): row = (lane * vmagic) >> 40
+  int tile;                   // 1: LDS-staged tiles (block = STEP_TILE, cpl = 1)
   int fface[6];               // per face: slot * 4 + mode of its winning slice BC
   float fval32[6][27];        // per face and direction: addend / constant (FP32 lattices)
   double fval64[6][27];       // the same for FP64 lattices
@@ -182,6 +185,7 @@ struct StepLaunch {
   int block;
   int variant;   // per-cell relaxation: 0 none, 1 LBGKTurbulence or the FP32 closure, 2 the matrix-core closure
   int cpl;       // cells per lane (2 or 4: the run variant; non-periodic, under 4 GiB, no closure)
+  int tile;      // 1: LDS-staged tiles of STEP_TILE cells (same restrictions, cpl 1, block STEP_TILE)
 };
 
 // wide: the populations span more than 4 GiB (64-bit per-lane addresses).

@@ -186,6 +186,7 @@ struct aslhip_lattice {
   bool force_wide = false;  // 64-bit addressing on any size (ASLHIP_TUNE_WIDE)
   int tune_remap = 4;  // chunk runs per XCD (measured best for graph chunks; the autotuner may choose others)
   int tune_cpl = 0;    // cells per lane along z: 0 automatic, else 1, 2 or 4 (ASLHIP_TUNE_CPL)
+  int tune_lds = 1;    // 1: stage tiles of populations through LDS (ASLHIP_TUNE_LDS)
   // face rule: per face slot * 4 + mode of the winning slice BC, and values
   int fface[6] = {0, 0, 0, 0, 0, 0};
   float fval32[6][27] = {};
@@ -606,21 +607,32 @@ bool is_wide(const aslhip_lattice *h) { return h->force_wide || h->qs_bytes * h-
 // Cells per lane the step kernel runs with: runs of 2 or 4 cells along z on
 // non-periodic lattices under 4 GiB without a per-cell relaxation, when nz is
 // a multiple of the run; else one cell per lane (the longest run that divides
-// nz).  Automatic: one cell per lane, measured fastest on the sphere channel.
+// nz).  Automatic: one cell per lane.
+// LDS-staged tiles: the default wherever they apply (non-periodic lattices
+// under 4 GiB without a per-cell relaxation) unless runs of 2 or 4 cells per
+// lane are asked for; measured fastest on the sphere channel.
+bool effective_lds(const aslhip_lattice *h) {
+  return h->tune_lds && h->tune_cpl <= 1 && !(h->periodic || is_wide(h) || h->has_closure || h->turbulence);
+}
+
 int effective_cpl(const aslhip_lattice *h) {
-  if (h->periodic || is_wide(h) || h->has_closure || h->turbulence) return 1;
+  if (h->periodic || is_wide(h) || h->has_closure || h->turbulence || effective_lds(h)) return 1;
   int c = h->tune_cpl;
   if (c == 0) c = 1;
   while (c > 1 && h->n[2] % c) c /= 2;
   return c;
 }
 
-// Workgroup size the step kernel runs with (runs of cells: at most STEP_RUN_BLOCK).
+// Workgroup size the step kernel runs with (runs of cells: at most
+// STEP_RUN_BLOCK; LDS-staged tiles: STEP_TILE).
 int step_workgroup(const aslhip_lattice *h) {
+  if (effective_lds(h)) return STEP_TILE;
   return effective_cpl(h) > 1 && h->tune_block > STEP_RUN_BLOCK ? STEP_RUN_BLOCK : h->tune_block;
 }
 
-// Kernel arguments of one step of h (flags filled in by the caller).
+// Kernel arguments of one step of h (flags filled in by the caller).  With
+// block and cpl 0 the lattice's own tuning (LDS-staged tiles included), else
+// one cell per lane in workgroups of that size.
 StepArgs step_args(const aslhip_lattice *h, int block = 0, int cpl = 0) {
   StepArgs a;
   std::memset(&a, 0, sizeof a);  // compared bytewise by upload_args
@@ -642,6 +654,9 @@ StepArgs step_args(const aslhip_lattice *h, int block = 0, int cpl = 0) {
   a.cpl = cpl > 0 ? cpl : effective_cpl(h);
   a.block = block > 0 ? block : h->tune_block;
   if (a.cpl > 1 && a.block > STEP_RUN_BLOCK) a.block = STEP_RUN_BLOCK;
+  a.tile = block <= 0 && cpl <= 0 && effective_lds(h) ? 1 : 0;
+  if (a.tile) a.block = STEP_TILE;
+
   a.nzv = h->n[2] / a.cpl;
   // lanes per plane: every cell of the rows (ghost columns included), or the runs of the interior
   const int lanes = a.cpl > 1 ? h->n[1] * a.nzv : h->n[1] * h->sy;
@@ -710,6 +725,7 @@ int launch_one(aslhip_lattice *h, unsigned fl, std::vector<hipEvent_t> *ev, int 
   // LBGKTurbulence takes precedence over a closure in the collision
   a.variant = h->turbulence ? 1 : (h->has_closure ? (h->closure_mfma ? 2 : 1) : 0);
   a.cpl = h->h_args[0].cpl;
+  a.tile = h->h_args[0].tile;
   if (ev) HIPCHK(hipEventRecord((*ev)[2 * s], h->stream));
   HIPCHK(launch_step(h->Q, h->dtype, h->next_odd, h->periodic, is_wide(h), a, h->stream));
   if (ev) HIPCHK(hipEventRecord((*ev)[2 * s + 1], h->stream));
@@ -1441,6 +1457,7 @@ int surface_impl(aslhip_lattice *h, const float *map, void *out, bool flux) {
     a.args = dpk;
     a.variant = 0;
     a.cpl = 1;
+    a.tile = 0;
     e = launch_peek(h->Q, h->dtype, h->next_odd, h->periodic, a, in, h->stream);
   }
   if (e == hipSuccess) {
@@ -1555,6 +1572,13 @@ int aslhip_set_tuning(aslhip_lattice *h, int knob, int value) {
     h->tune_block = value;
     return 0;
   }
+  if (knob == ASLHIP_TUNE_LDS) {
+    if (value != 0 && value != 1) return fail(ASLHIP_EINVAL, "ASL ERROR: ASLHIP_TUNE_LDS takes 0 or 1");
+    if (int r = flush_deferred(h)) return r;
+    if (value != h->tune_lds) drop_graphs(h);  // captured launches name the variant and grid
+    h->tune_lds = value;
+    return 0;
+  }
   return fail(ASLHIP_EINVAL, "ASL ERROR: unknown tuning knob %d", knob);
 }
 
@@ -1566,6 +1590,7 @@ int aslhip_tuning_value(aslhip_lattice *h, int knob, int *value) {
     case ASLHIP_TUNE_BLOCK: *value = step_workgroup(h); return 0;
     case ASLHIP_TUNE_WIDE: *value = is_wide(h) ? 1 : 0; return 0;
     case ASLHIP_TUNE_CPL: *value = effective_cpl(h); return 0;
+    case ASLHIP_TUNE_LDS: *value = effective_lds(h) ? 1 : 0; return 0;
     default: return fail(ASLHIP_EINVAL, "ASL ERROR: unknown tuning knob %d", knob);
   }
 }
@@ -1574,29 +1599,43 @@ int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_re
   if (int r = check(h)) return r;
   if (nsteps < 1) return fail(ASLHIP_EINVAL, "ASL ERROR: autotune needs at least one step per candidate");
   nsteps += nsteps & 1;
-  const int blocks[3] = {256, 512, 1024};
+  const int remaps[6] = {0, 2, 4, 8, 16, 64};
   double best = 1e300;
-  int bb = h->tune_block, br = h->tune_remap, bc = h->tune_cpl;
-  std::vector<int> cpls;  // the distinct cells-per-lane values this lattice can run
+  int bb = h->tune_block, br = h->tune_remap, bc = h->tune_cpl, bl = h->tune_lds;
+  auto trial = [&](int lds, int cpl, int remap, int blk) -> int {
+    if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_LDS, lds)) return r;
+    if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_CPL, cpl)) return r;
+    if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_XCD_REMAP, remap)) return r;
+    if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_BLOCK, blk)) return r;
+    double tot = 0., ker = 0.;
+    if (int r = aslhip_time_steps(h, nsteps, 0u, &tot, &ker, nullptr)) return r;
+    if (ker < best) {
+      best = ker;
+      bb = blk;
+      br = remap;
+      bc = cpl;
+      bl = lds;
+    }
+    return 0;
+  };
+  // LDS-staged tiles (fixed workgroup) where the lattice can run them
+  if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_CPL, 0)) return r;
+  if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_LDS, 1)) return r;
+  if (effective_lds(h))
+    for (int remap : remaps)
+      if (int r = trial(1, 0, remap, h->tune_block)) return r;
+  // direct: the distinct cells-per-lane values this lattice can run
+  if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_LDS, 0)) return r;
+  std::vector<int> cpls;
   for (int c : {1, 2, 4}) {
     if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_CPL, c)) return r;
     if (effective_cpl(h) == c) cpls.push_back(c);
   }
   for (int cpl : cpls)
-    for (int remap : {0, 2, 4, 8, 16, 64})
-      for (int blk : blocks) {
-        if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_CPL, cpl)) return r;
-        if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_XCD_REMAP, remap)) return r;
-        if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_BLOCK, blk)) return r;
-        double tot = 0., ker = 0.;
-        if (int r = aslhip_time_steps(h, nsteps, 0u, &tot, &ker, nullptr)) return r;
-        if (ker < best) {
-          best = ker;
-          bb = blk;
-          br = remap;
-          bc = cpl;
-        }
-      }
+    for (int remap : remaps)
+      for (int blk : {256, 512, 1024})
+        if (int r = trial(0, cpl, remap, blk)) return r;
+  if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_LDS, bl)) return r;
   if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_CPL, bc)) return r;
   if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_XCD_REMAP, br)) return r;
   if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_BLOCK, bb)) return r;

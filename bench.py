@@ -364,20 +364,23 @@ def launch_study():
     L.step(20, write_macro=False)
     counted = L.stats()["counted"]
     byts = 152.0 * counted + 1.0 * int(np.prod(SIZE))
-    for cpl in (1, 2, 4):
+    # cells per lane 1, 2, 4, then "lds": LDS-staged tiles of 256 cells
+    for cpl in (1, 2, 4, "lds"):
         for remap in (0, 4, 16, 64):
             for block in (256, 512, 1024):
-                if cpl > 1 and block > 512:
+                if cpl != 1 and block > 512 or cpl == "lds" and block != 256:
                     continue
-                L.set_tuning(_lib.TUNE_CPL, cpl)
+                L.set_tuning(_lib.TUNE_LDS, 1 if cpl == "lds" else 0)
+                L.set_tuning(_lib.TUNE_CPL, 1 if cpl == "lds" else cpl)
                 L.set_tuning(_lib.TUNE_XCD_REMAP, remap)
                 L.set_tuning(_lib.TUNE_BLOCK, block)
                 L.time_steps(20, write_macro=False)
                 _, _, each = L.time_steps(200, write_macro=False, each=True)
                 med = float(np.median(each))
                 print(json.dumps({"study": "step-kernel launch settings, 256x128x128 sphere channel",
-                                  "cells_per_lane": L.tuning_value(_lib.TUNE_CPL), "xcd_run": remap,
-                                  "workgroup": block, "kernel_ms_median": round(med, 5),
+                                  "cells_per_lane": L.tuning_value(_lib.TUNE_CPL),
+                                  "lds_tiles": L.tuning_value(_lib.TUNE_LDS), "xcd_run": remap,
+                                  "workgroup": L.tuning_value(_lib.TUNE_BLOCK), "kernel_ms_median": round(med, 5),
                                   "kernel_ms_even": round(float(np.median(each[0::2])), 5),
                                   "kernel_ms_odd": round(float(np.median(each[1::2])), 5),
                                   "tbps_algorithmic": round(byts / med / 1e9, 3),
@@ -451,79 +454,12 @@ def batch_study(steps, warmup, group_mb=None):
 
 
 def closure_study(nc, tau_fine, chunk, snaps, eval_chunks, target="total"):
-    """Coarse N^3 with and without the learned closure against the 4x finer run.
-
-    The network is trained, and its output scale chosen, on one realisation of
-    the initial field (seed + 1); the comparison runs on another (seed).
-    """
+    """Coarse N^3 with and without the learned closure against the 4x finer run
+    (``asl_amd.closure.study``): one JSON line."""
     from asl_amd import closure
-    t0 = time.perf_counter()
-    log = lambda m: print(m, flush=True)  # noqa: E731
-    tr = closure.Experiment(n_coarse=nc, tau_fine=tau_fine, seed=SEED + 1)
-    log("# closure study: coarse %d^3, fine %d^3, tau_fine %.4f; training field ready (%.1f s)"
-        % (nc, tr.nf, tau_fine, time.perf_counter() - t0))
-    x, t = tr.training_data(chunk, snaps, log=log)
-    tt = time.perf_counter()
-    params, mu, sc, out, losses = closure.train(x, t, closure.init_params(1), steps=4000, batch=16384, lr=2e-3)
-    train_s = time.perf_counter() - tt
-    log("# trained on %d samples in %.1f s: loss %.4f -> %.4f" % (t.size, train_s, losses[:20].mean(),
-                                                                 losses[-20:].mean()))
-    del x, t
-    # output scale (0 = plain LBGK) chosen on the training field, over the
-    # evaluation horizon: against the fine run's kinetic-energy decay
-    # (target "total") or the box-filtered fine kinetic energy ("filtered")
-    if target == "total":
-        F = tr.fine()
-        ke_tr, _ = tr.ke_history(F, chunk, eval_chunks, fine=True)
-        del F
-    else:
-        ke_tr = tr.filtered_ke_history(chunk, eval_chunks)
-    calib = {}
-    for a in (0.0, 0.1, 0.2, 0.35, 0.5, 0.75, 1.0, 1.5, 2.0):
-        C = tr.coarse()
-        if a > 0:
-            C.set_closure(params, mu, sc, out * a)
-        ke_a, _ = tr.ke_history(C, chunk, eval_chunks)
-        if not np.isfinite(ke_a).all():
-            calib[a] = float("inf")
-        elif target == "total":
-            calib[a] = float(np.abs((ke_a / ke_a[0]) / (ke_tr / ke_tr[0]) - 1.0).max())
-        else:
-            calib[a] = float(np.abs(ke_a / ke_tr - 1.0).max())
-    alpha = min(calib, key=calib.get)
-    log("# output scale on the training field (%s kinetic energy): %s -> %s" % (target, calib, alpha))
-    del tr
-    ex = closure.Experiment(n_coarse=nc, tau_fine=tau_fine, seed=SEED)
-    F = ex.fine()
-    ke_f, wall_f = ex.ke_history(F, chunk, eval_chunks, fine=True)
-    del F
-    log("# fine run done (%.1f s)" % wall_f)
-    ke_ff = ex.filtered_ke_history(chunk, eval_chunks)
-    log("# filtered fine kinetic energy done")
-    P = ex.coarse()
-    ke_p, wall_p = ex.ke_history(P, chunk, eval_chunks)
-    C = ex.coarse()
-    C.set_closure(params, mu, sc, out * max(alpha, 1e-30))
-    ke_c, wall_c = ex.ke_history(C, chunk, eval_chunks)
-
-    def err(ke):  # against the fine run's total kinetic energy, both normalised by their start
-        return float(np.abs((ke / ke[0]) / (ke_f / ke_f[0]) - 1.0).max())
-
-    def err_filtered(ke):  # against the box-filtered fine run (the field the coarse lattice represents)
-        return float(np.abs(ke / ke_ff - 1.0).max())
-    print(json.dumps({"study": "learned closure, D3Q19 decaying turbulence", "coarse": nc, "fine": ex.nf,
-                      "tau_fine": tau_fine, "scale_chosen_against": target, "tau_coarse": 0.5 + 3 * ex.nu_c, "coarse_steps_per_chunk": chunk,
-                      "train_snapshots": snaps, "eval_chunks": eval_chunks, "output_scale_factor": alpha,
-                      "calibration_errors": {str(k): v for k, v in calib.items()},
-                      "ke_fine": ke_f.tolist(), "ke_fine_filtered": ke_ff.tolist(), "ke_coarse_plain": ke_p.tolist(),
-                      "ke_coarse_closure": ke_c.tolist(),
-                      "max_rel_ke_decay_error_plain": err(ke_p), "max_rel_ke_decay_error_closure": err(ke_c),
-                      "max_rel_ke_error_vs_filtered_plain": err_filtered(ke_p),
-                      "max_rel_ke_error_vs_filtered_closure": err_filtered(ke_c),
-                      "wall_s_fine": wall_f, "wall_s_coarse_plain": wall_p, "wall_s_coarse_closure": wall_c,
-                      "speedup_closure_vs_fine": wall_f / wall_c, "train_s": train_s,
-                      "train_loss_first": float(losses[:20].mean()), "train_loss_last": float(losses[-20:].mean())}),
-          flush=True)
+    r = closure.study(nc, tau_fine, chunk=chunk, snapshots=snaps, eval_chunks=eval_chunks, target=target,
+                      seed=SEED, log=lambda m: print(m, flush=True))
+    print(json.dumps(r), flush=True)
 
 
 def variant_study(n, steps=500):
@@ -580,6 +516,9 @@ def main():
                     help="XCD run length of the step kernel's chunks (default: the library's)")
     ap.add_argument("--cpl", type=int, default=-1, metavar="C",
                     help="cells per lane of the step kernel (1, 2, 4; default: the library's choice)")
+    ap.add_argument("--lds", type=int, default=-1, metavar="B",
+                    help="1: the step kernel stages tiles of 256 cells through LDS; 0: direct "
+                         "(default: the library's choice)")
     ap.add_argument("--launch-study", action="store_true",
                     help="step-kernel duration on the sphere channel for every cells-per-lane x XCD run x workgroup")
     ap.add_argument("--autotune", action="store_true",
@@ -655,6 +594,8 @@ def main():
     # timed real steps (they advance the flow like the warm-up steps)
     if args.cpl >= 0:
         L.set_tuning(_lib.TUNE_CPL, args.cpl)
+    if args.lds >= 0:
+        L.set_tuning(_lib.TUNE_LDS, args.lds)
     block, remap = L.autotune(20) if args.autotune else L.tuning()
     if args.remap >= 0:
         L.set_tuning(_lib.TUNE_XCD_REMAP, args.remap)
@@ -711,7 +652,7 @@ def main():
                      "kernel_ms_median": round(float(np.median(each)), 5), "kernel_launches_timed": KERNEL_LAUNCHES,
                      "bytes_per_launch": bytes_per_launch, "graph_ms_per_step": round(ms_graph / args.steps, 5),
                      "launch": {"workgroup": L.tuning_value(_lib.TUNE_BLOCK), "xcd_run": remap,
-                                "cells_per_lane": cpl}},
+                                "cells_per_lane": cpl, "lds_tiles": L.tuning_value(_lib.TUNE_LDS)}},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -364,21 +364,31 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
  * automatic (default: one cell per lane).  Runs of 2 or 4 cells apply to
  * non-periodic lattices under 4 GiB without a per-cell relaxation whose nz
  * is a multiple of the run (else the longest run that divides nz, down to
- * one cell); results are bit for bit the same for every value. */
+ * one cell) and take precedence over ASLHIP_TUNE_LDS; results are bit for
+ * bit the same for every value. */
 #define ASLHIP_TUNE_CPL 5
+/* Lattices: 1 = the step kernel stages the populations of tiles of 256
+ * consecutive cells through LDS (16-byte loads and stores of each
+ * direction's slots, workgroup 256; default); 0 = direct, one cell per lane
+ * (or the runs ASLHIP_TUNE_CPL asks for).  Tiles apply to the same lattices
+ * as runs of cells when ASLHIP_TUNE_CPL is 0 or 1; results are bit for bit
+ * the same either way. */
+#define ASLHIP_TUNE_LDS 6
 int aslhip_set_tuning(aslhip_lattice *h, int knob, int value);
 /* The value a tuning knob has now (ASLHIP_TUNE_CPL: the cells per lane the
- * step kernel runs with). */
+ * step kernel runs with; ASLHIP_TUNE_LDS: 1 if it runs LDS-staged tiles). */
 int aslhip_tuning_value(aslhip_lattice *h, int knob, int *value);
 /* The workgroup size the step kernel runs with (runs of 2 or 4 cells per lane:
- * at most 512) and the XCD run length (either may be NULL). */
+ * at most 512; LDS-staged tiles: 256) and the XCD run length (either may be
+ * NULL). */
 int aslhip_get_tuning(aslhip_lattice *h, int *block, int *remap);
 
 /* The autotuner: times nsteps (rounded up to even) real steps of the lattice
- * for every candidate setting (cells per lane 1/2/4 where the lattice can run
- * them x XCD runs 0/2/4/8/16/64 x workgroup 256/512/1024) with HIP events,
- * keeps the fastest and returns its workgroup size and XCD run length (either
- * pointer may be NULL; the cells per lane: aslhip_tuning_value).
+ * for every candidate setting (LDS-staged tiles x XCD runs 0/2/4/8/16/64
+ * where the lattice can run them; direct with cells per lane 1/2/4 where it
+ * can run them x XCD runs 0/2/4/8/16/64 x workgroup 256/512/1024) with HIP
+ * events, keeps the fastest and returns its workgroup size and XCD run
+ * length (either pointer may be NULL; the variant: aslhip_tuning_value).
  * The trial steps advance the simulation like any other steps. */
 int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_remap);
 

@@ -2,7 +2,11 @@
 // ZE, no-slip elsewhere, lid executed first as in examples/flow/flow2.cc)
 // through the asl:: compatibility layer on the GPU, 1000 FP32 steps, against
 // the CPU oracle.  Returns EXIT_SUCCESS if rho and u agree within 1e-5 (the
-// port's contract) and reports whether they are bit-identical.
+// port's contract) and reports whether they are bit-identical.  Also reads
+// the velocity through getVelocity()->deviceData() (the device copy a coupled
+// solver on the GPU consumes) and compares it with the host view.
+#include <hip/hip_runtime_api.h>
+
 #include <cmath>
 #include <cstdlib>
 #include <iostream>
@@ -71,6 +75,18 @@ int main(int argc, char *argv[]) {
       }
   std::cout << "cavity " << n << "^3 x " << steps << " FP32 via asl::LBGK: max|drho| " << dr << ", max|du| " << du
             << " (max|u| " << umax << "), bit-identical: " << (bitwise ? "yes" : "no") << std::endl;
-  const bool ok = dr <= 1e-5 && du <= 1e-5 * umax && umax > 1e-3;
+  // the device view: component-major, ghost-inclusive, FP32 on an FP32 lattice
+  const void *dev = vel->deviceData(nullptr);
+  std::vector<float> du_dev(3 * len);
+  bool same_dev = dev != nullptr &&
+                  hipMemcpy(du_dev.data(), dev, du_dev.size() * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
+  for (int x = 1; x <= n && same_dev; ++x)
+    for (int y = 1; y <= n; ++y)
+      for (int z = 1; z <= n; ++z) {
+        const long j = (x * S + y) * S + z;
+        for (int d = 0; d < 3; ++d) same_dev = same_dev && du_dev[d * len + j] == float(vel->component(d)[j]);
+      }
+  std::cout << "device velocity view equals host view: " << (same_dev ? "yes" : "no") << std::endl;
+  const bool ok = dr <= 1e-5 && du <= 1e-5 * umax && umax > 1e-3 && same_dev;
   return ok ? EXIT_SUCCESS : EXIT_FAILURE;
 }

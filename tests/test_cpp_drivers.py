@@ -33,6 +33,7 @@ def test_cpp_cavity_64_1000():
     p = _run("cavity_parity", 64, 1000)
     assert p.returncode == 0
     assert "bit-identical: yes" in p.stdout
+    assert "device velocity view equals host view: yes" in p.stdout
 
 
 @pytest.mark.gpu

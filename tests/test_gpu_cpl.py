@@ -1,9 +1,11 @@
-"""Runs of 2 or 4 cells per lane (the step kernel's vector variant) end bit for
-bit where one cell per lane ends: same addresses, same link resolution, same
-collision per cell.  Covered: every slice BC kind on every face in both
-execute orders, map BCs (no-slip, pressure, velocity with its boundary-link
-kernel), the rotating frame, incompressible mode, D3Q15/19/27, all storage
-types, graph replays, and sizes whose nz does not divide by the run.
+"""Runs of 2 or 4 cells per lane (the step kernel's vector variant) and
+LDS-staged tiles of 256 cells end bit for bit where one cell per lane ends:
+same addresses, same link resolution, same collision per cell.  Covered:
+every slice BC kind on every face in both execute orders, map BCs (no-slip,
+pressure, velocity with its boundary-link kernel), the rotating frame,
+incompressible mode, D3Q15/19/27, all storage types, graph replays, sizes
+whose nz does not divide by the run, and planes of several tiles whose
+boundaries cut rows (bounce-back partners outside the tile).
 """
 import numpy as np
 import pytest
@@ -29,8 +31,13 @@ def sphere(S, c, r):
 
 
 def build(size, q, dtype, cpl, case):
+    """cpl: cells per lane (1: the direct kernel), or "lds" for LDS-staged tiles."""
     L = asl_amd.Lattice(size, q, dtype=dtype, nu=0.05)
-    L.set_tuning(_lib.TUNE_CPL, cpl)
+    if cpl == "lds":
+        L.set_tuning(_lib.TUNE_LDS, 1)
+    else:
+        L.set_tuning(_lib.TUNE_LDS, 0)
+        L.set_tuning(_lib.TUNE_CPL, cpl)
     S = L.S
     if case == "slices":
         L.init_equilibrium(1.0, (0.02, -0.01, 0.005))
@@ -77,11 +84,26 @@ def test_runs_equal_single_cells(case, q):
     ref = build(size, q, "f32", 1, case)
     ref.step(41)
     want = state(ref)
-    for cpl in (2, 4):
+    for cpl in (2, 4, "lds"):
         L = build(size, q, "f32", cpl, case)
-        assert L.tuning_value(_lib.TUNE_CPL) == cpl
+        if cpl == "lds":
+            assert L.tuning_value(_lib.TUNE_LDS) == 1 and L.tuning()[0] == 256
+        else:
+            assert L.tuning_value(_lib.TUNE_CPL) == cpl
         L.step(41)
         assert same(state(L), want), (case, q, cpl)
+
+
+@pytest.mark.parametrize("case", ["slices", "maps", "frame"])
+@pytest.mark.parametrize("q", [15, 19, 27])
+def test_tiles_across_rows_equal_single_cells(case, q):
+    # planes of 23 rows of 32 cells: three tiles per plane (256, 256, 224 cells)
+    size = (20, 23, 30)
+    ref = build(size, q, "f32", 1, case)
+    ref.step(23)
+    L = build(size, q, "f32", "lds", case)
+    L.step(23)
+    assert same(state(L), state(ref)), (case, q)
 
 
 @pytest.mark.parametrize("dtype", ["f64", "bf16", "f16"])
@@ -90,7 +112,7 @@ def test_runs_equal_single_cells_storage(dtype):
     ref = build(size, 19, dtype, 1, "maps")
     ref.step(33)
     want = state(ref)
-    for cpl in (2, 4):
+    for cpl in (2, 4, "lds"):
         L = build(size, 19, dtype, cpl, "maps")
         L.step(33)
         assert same(state(L), want), (dtype, cpl)
@@ -108,12 +130,20 @@ def test_runs_under_graph_replay_and_tuning_changes():
     L.set_tuning(_lib.TUNE_CPL, 0)
     L.step(100, graph=True)
     assert same(state(L), state(ref))
+    L.set_tuning(_lib.TUNE_LDS, 1)
+    assert L.tuning_value(_lib.TUNE_LDS) == 1
+    L.step(131, graph=True)
+    L.set_tuning(_lib.TUNE_LDS, 0)
+    L.step(20, graph=True)
+    ref.step(151)
+    assert same(state(L), state(ref))
 
 
 def test_run_length_follows_nz():
     for nz, want in ((12, 4), (10, 2), (9, 1)):
         L = asl_amd.Lattice((6, 5, nz), 19, nu=0.1)
-        assert L.tuning_value(_lib.TUNE_CPL) == 1  # the default
+        # the default: LDS-staged tiles, one cell per lane
+        assert L.tuning_value(_lib.TUNE_CPL) == 1 and L.tuning_value(_lib.TUNE_LDS) == 1
         L.set_tuning(_lib.TUNE_CPL, 4)
         assert L.tuning_value(_lib.TUNE_CPL) == want
         L.set_tuning(_lib.TUNE_CPL, 2)
@@ -123,3 +153,15 @@ def test_run_length_follows_nz():
     assert L.tuning_value(_lib.TUNE_CPL) == 1  # periodic wrap: one cell per lane
     with pytest.raises(asl_amd.ASLError):
         L.set_tuning(_lib.TUNE_CPL, 3)
+    L.set_tuning(_lib.TUNE_LDS, 1)
+    assert L.tuning_value(_lib.TUNE_LDS) == 0  # periodic wrap: direct
+    with pytest.raises(asl_amd.ASLError):
+        L.set_tuning(_lib.TUNE_LDS, 2)
+    L = asl_amd.Lattice((6, 5, 12), 19, nu=0.1)
+    L.set_tuning(_lib.TUNE_CPL, 4)  # runs take precedence over tiles
+    assert L.tuning_value(_lib.TUNE_LDS) == 0 and L.tuning_value(_lib.TUNE_CPL) == 4
+    L.set_tuning(_lib.TUNE_CPL, 0)
+    assert L.tuning_value(_lib.TUNE_LDS) == 1 and L.tuning_value(_lib.TUNE_CPL) == 1
+    assert L.tuning()[0] == 256
+    L.set_tuning(_lib.TUNE_LDS, 0)
+    assert L.tuning() == (512, 4)
