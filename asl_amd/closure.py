@@ -153,6 +153,9 @@ class Experiment:
     """Fine / coarse decaying-turbulence runs sharing one initial field."""
 
     def __init__(self, n_coarse=128, tau_fine=0.55, u_rms=0.03, seed=20261202, r=4, ce_init=False):
+        """ce_init: False (both lattices start from equilibrium), True (both
+        from equilibrium + the first Chapman-Enskog term) or "coarse" (only
+        the coarse lattice; the fine run resolves its own start)."""
         self.nc, self.r = n_coarse, r
         self.nf = n_coarse * r
         self.nu_f = (tau_fine - 0.5) / 3.0
@@ -175,7 +178,7 @@ class Experiment:
 
     def fine(self):
         L = Lattice((self.nf,) * 3, 19, dtype="f32", periodic=True, nu=self.nu_f)
-        if self.ce_init:
+        if self.ce_init is True:
             L.write_f(with_ghosts(populations(self.rho0, self.u0, 3.0 * self.nu_f + 0.5)))
         else:
             L.init_equilibrium(with_ghosts(self.rho0), with_ghosts(self.u0))
@@ -281,7 +284,7 @@ def _ke_error(ke, ke_ref):
 
 
 def study(n_coarse, tau_fine, chunk=50, snapshots=16, eval_chunks=16, target="total", seed=20261202,
-          train_steps=4000, batch=16384, grid=SCALE_GRID, log=print):
+          train_steps=4000, batch=16384, grid=SCALE_GRID, ce_init=False, log=print):
     """Coarse n^3 with and without the learned closure against the 4x finer run.
 
     The network is trained, and its output scale chosen from `grid`, on one
@@ -289,10 +292,11 @@ def study(n_coarse, tau_fine, chunk=50, snapshots=16, eval_chunks=16, target="to
     kinetic-energy decay (target "total") or the box-filtered fine kinetic
     energy ("filtered"), over the evaluation horizon of eval_chunks chunks of
     `chunk` coarse steps.  The comparison runs on another realisation (seed).
-    Returns a dict of the histories, errors and wall times.
+    ce_init: the lattices' start (Experiment).  Returns a dict of the
+    histories, errors and wall times.
     """
     t0 = time.perf_counter()
-    tr = Experiment(n_coarse=n_coarse, tau_fine=tau_fine, seed=seed + 1)
+    tr = Experiment(n_coarse=n_coarse, tau_fine=tau_fine, seed=seed + 1, ce_init=ce_init)
     log("# closure study: coarse %d^3, fine %d^3, tau_fine %.4f; training field ready (%.1f s)"
         % (n_coarse, tr.nf, tau_fine, time.perf_counter() - t0))
     x, t = tr.training_data(chunk, snapshots, log=log)
@@ -319,7 +323,7 @@ def study(n_coarse, tau_fine, chunk=50, snapshots=16, eval_chunks=16, target="to
     alpha = min(calib, key=calib.get)
     log("# output scale on the training field (%s kinetic energy): %s -> %s" % (target, calib, alpha))
     del tr
-    ex = Experiment(n_coarse=n_coarse, tau_fine=tau_fine, seed=seed)
+    ex = Experiment(n_coarse=n_coarse, tau_fine=tau_fine, seed=seed, ce_init=ce_init)
     F = ex.fine()
     ke_f, wall_f = ex.ke_history(F, chunk, eval_chunks, fine=True)
     del F
@@ -335,6 +339,8 @@ def study(n_coarse, tau_fine, chunk=50, snapshots=16, eval_chunks=16, target="to
     del C
     return {"study": "learned closure, D3Q19 decaying turbulence", "coarse": n_coarse, "fine": ex.nf,
             "tau_fine": tau_fine, "scale_chosen_against": target, "tau_coarse": 0.5 + 3 * ex.nu_c,
+            "start": {False: "equilibrium", True: "equilibrium + Chapman-Enskog term",
+                      "coarse": "coarse: equilibrium + Chapman-Enskog term; fine: equilibrium"}[ce_init],
             "coarse_steps_per_chunk": chunk, "train_snapshots": snapshots, "eval_chunks": eval_chunks,
             "output_scale_factor": alpha, "calibration_errors": {str(k): v for k, v in calib.items()},
             "ke_fine": ke_f.tolist(), "ke_fine_filtered": ke_ff.tolist(), "ke_coarse_plain": ke_p.tolist(),

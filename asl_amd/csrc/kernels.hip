@@ -703,6 +703,36 @@ __device__ __forceinline__ void step_body_run(const StepArgs &a, const int L) {
   }
 }
 
+// LDS slot of a raw storage element: the element itself, or its bits in a
+// 32-bit word for 16-bit storage.
+template <class S>
+struct LdsSlot {
+  using type = S;
+  static __device__ __forceinline__ S put(S x) { return x; }
+  static __device__ __forceinline__ S get(S x) { return x; }
+};
+template <>
+struct LdsSlot<unsigned short> {
+  using type = unsigned;
+  static __device__ __forceinline__ unsigned put(unsigned short x) { return x; }
+  static __device__ __forceinline__ unsigned short get(unsigned x) { return static_cast<unsigned short>(x); }
+};
+template <>
+struct LdsSlot<_Float16> {
+  using type = unsigned;
+  static __device__ __forceinline__ unsigned put(_Float16 x) {
+    unsigned short b;
+    __builtin_memcpy(&b, &x, 2);
+    return b;
+  }
+  static __device__ __forceinline__ _Float16 get(unsigned x) {
+    const unsigned short b = static_cast<unsigned short>(x);
+    _Float16 r;
+    __builtin_memcpy(&r, &b, 2);
+    return r;
+  }
+};
+
 // Direction components packed two bits each ((c + 1) << 2k), so that the
 // offset of a direction chosen at run time is a few scalar instructions.
 template <int Q>
@@ -754,9 +784,13 @@ __device__ __forceinline__ void step_body_tile(const StepArgs &a, const int L) {
   constexpr int EPL = TB / 64;            // row elements per lane
   constexpr unsigned ES = sizeof(S);
   static_assert(TB % 64 == 0 && (EPL & (EPL - 1)) == 0, "tile: a power-of-two multiple of 64 cells");
+  // LDS slots hold the raw storage bits; 16-bit storage gets 32-bit slots
+  // (sub-dword LDS writes from neighbouring lanes collide in a bank)
+  using LS = typename LdsSlot<S>::type;
   typedef S V4 __attribute__((ext_vector_type(EPL), aligned(sizeof(S))));  // global: element alignment
-  typedef S L4 __attribute__((ext_vector_type(EPL)));                       // LDS: EPL elements aligned
-  __shared__ __attribute__((aligned(16 * sizeof(S)))) S row[Q][TB];
+  typedef LS L4 __attribute__((ext_vector_type(EPL)));                      // LDS: EPL slots aligned
+  __shared__ __attribute__((aligned(16 * sizeof(LS)))) LS row[Q][TB];
+  auto rd = [&](int j, int col) -> C { return CD::dec(LdsSlot<S>::get(row[j][col])); };
   const int t = static_cast<int>(threadIdx.x);
   const int plane = L / a.bpp;
   const int chunk = L - plane * a.bpp;
@@ -819,7 +853,8 @@ __device__ __forceinline__ void step_body_tile(const StepArgs &a, const int L) {
       const int j = w + i * NW;
       if (j < Q) {
         L4 v4;
-        __builtin_memcpy(&v4, &xr[i], sizeof v4);
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) v4[q] = LdsSlot<S>::put(xr[i][q]);
         *reinterpret_cast<L4 *>(&row[j][e]) = v4;
       }
     }
@@ -879,7 +914,7 @@ __device__ __forceinline__ void step_body_tile(const StepArgs &a, const int L) {
   const int tc = in_tile ? t : 0;
   C v[Q];
 #pragma unroll
-  for (int k = 0; k < Q; ++k) v[k] = CD::dec(row[in_row(k)][tc]);
+  for (int k = 0; k < Q; ++k) v[k] = rd(in_row(k), tc);
   if (faces) {
 #pragma unroll
     for (int k = 1; k < Q; ++k) {
@@ -887,7 +922,7 @@ __device__ __forceinline__ void step_body_tile(const StepArgs &a, const int L) {
       C val = C(0);
       (void)winner(k, val);
       if ((bmask >> k) & 1u) {
-        const C o = other_in_tile(k) ? CD::dec(row[out_row(k)][other_col(k)]) : side[k];
+        const C o = other_in_tile(k) ? rd(out_row(k), other_col(k)) : side[k];
         v[k] = o + val;
       } else {
         v[k] = val;
@@ -906,9 +941,9 @@ __device__ __forceinline__ void step_body_tile(const StepArgs &a, const int L) {
     for (int k = 1; k < Q; ++k) {
       const int slot = raw ? 0 : rec.slot[k];
       const unsigned char md = slot ? a.mode[slot] : static_cast<unsigned char>(MODE_NONE);
-      C xk = CD::dec(row[in_row(k)][t]);
+      C xk = rd(in_row(k), t);
       if (md == MODE_BOUNCE)
-        xk = other_in_tile(k) ? CD::dec(row[out_row(k)][other_col(k)]) : other_global(k);
+        xk = other_in_tile(k) ? rd(out_row(k), other_col(k)) : other_global(k);
       const C sv = val[slot * Q + k];
       v[k] = md == MODE_BOUNCE ? xk + sv : (md == MODE_CONST ? sv : xk);
     }
@@ -927,7 +962,7 @@ __device__ __forceinline__ void step_body_tile(const StepArgs &a, const int L) {
   if (live) {
 #pragma unroll
     for (int k = 0; k < Q; ++k)
-      if (!((skipm >> k) & 1u)) row[out_row(k)][t] = CD::enc(v[k]);
+      if (!((skipm >> k) & 1u)) row[out_row(k)][t] = LdsSlot<S>::put(CD::enc(v[k]));
     if (a.flags & STEP_F_RHO) *gstore<M>(static_cast<char *>(a.rho) + sizeof(M) * idx) = M(r);
     if (a.flags & STEP_F_VEL) {
       char *u = static_cast<char *>(a.u);
@@ -949,13 +984,15 @@ __device__ __forceinline__ void step_body_tile(const StepArgs &a, const int L) {
       if (e + EPL <= n && g >= 0 && g + EPL <= len) {
         const L4 l4 = *reinterpret_cast<const L4 *>(&row[j][e]);
         V4 x4;
-        __builtin_memcpy(&x4, &l4, sizeof x4);
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) x4[q] = LdsSlot<S>::get(l4[q]);
         *gstore<V4>(fw + static_cast<unsigned>(j) * a.qs32 + static_cast<unsigned>(g) * ES) = x4;
       } else {
 #pragma unroll
         for (int q = 0; q < EPL; ++q)
           if (e + q < n && g + q >= 0 && g + q < len)
-            *gstore<S>(fw + static_cast<unsigned>(j) * a.qs32 + static_cast<unsigned>(g + q) * ES) = row[j][e + q];
+            *gstore<S>(fw + static_cast<unsigned>(j) * a.qs32 + static_cast<unsigned>(g + q) * ES) =
+                LdsSlot<S>::get(row[j][e + q]);
       }
     }
   }

@@ -186,7 +186,7 @@ struct aslhip_lattice {
   bool force_wide = false;  // 64-bit addressing on any size (ASLHIP_TUNE_WIDE)
   int tune_remap = 4;  // chunk runs per XCD (measured best for graph chunks; the autotuner may choose others)
   int tune_cpl = 0;    // cells per lane along z: 0 automatic, else 1, 2 or 4 (ASLHIP_TUNE_CPL)
-  int tune_lds = 1;    // 1: stage tiles of populations through LDS (ASLHIP_TUNE_LDS)
+  int tune_lds = -1;   // LDS-staged tiles: 1 on, 0 off, -1 automatic (ASLHIP_TUNE_LDS)
   // face rule: per face slot * 4 + mode of the winning slice BC, and values
   int fface[6] = {0, 0, 0, 0, 0, 0};
   float fval32[6][27] = {};
@@ -608,11 +608,13 @@ bool is_wide(const aslhip_lattice *h) { return h->force_wide || h->qs_bytes * h-
 // non-periodic lattices under 4 GiB without a per-cell relaxation, when nz is
 // a multiple of the run; else one cell per lane (the longest run that divides
 // nz).  Automatic: one cell per lane.
-// LDS-staged tiles: the default wherever they apply (non-periodic lattices
-// under 4 GiB without a per-cell relaxation) unless runs of 2 or 4 cells per
-// lane are asked for; measured fastest on the sphere channel.
+// LDS-staged tiles apply to non-periodic lattices under 4 GiB without a
+// per-cell relaxation unless runs of 2 or 4 cells per lane are asked for.
+// Automatic: on for FP32 storage (measured fastest on the sphere channel),
+// off for FP16 / BF16 / FP64 storage (the direct kernel is faster there).
 bool effective_lds(const aslhip_lattice *h) {
-  return h->tune_lds && h->tune_cpl <= 1 && !(h->periodic || is_wide(h) || h->has_closure || h->turbulence);
+  const bool on = h->tune_lds < 0 ? h->dtype == ASLHIP_F32 : h->tune_lds != 0;
+  return on && h->tune_cpl <= 1 && !(h->periodic || is_wide(h) || h->has_closure || h->turbulence);
 }
 
 int effective_cpl(const aslhip_lattice *h) {
@@ -1601,7 +1603,7 @@ int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_re
   nsteps += nsteps & 1;
   const int remaps[6] = {0, 2, 4, 8, 16, 64};
   double best = 1e300;
-  int bb = h->tune_block, br = h->tune_remap, bc = h->tune_cpl, bl = h->tune_lds;
+  int bb = h->tune_block, br = h->tune_remap, bc = h->tune_cpl, bl = effective_lds(h) ? 1 : 0;
   auto trial = [&](int lds, int cpl, int remap, int blk) -> int {
     if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_LDS, lds)) return r;
     if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_CPL, cpl)) return r;

@@ -38,7 +38,8 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--autotun
            (D3Q27 periodic Taylor-Green vortex on N^3, tau = 0.5005: FP32, FP16
            and BF16 storage against FP64; one JSON line per storage type)
        python bench.py --closure-study N [--tau-fine T] [--chunk C] [--train-snapshots K]
-           [--eval-chunks E] [--closure-target total|filtered]  (learned closure: N^3
+           [--eval-chunks E] [--closure-target total|filtered] [--ce-start none|coarse|both]
+           (learned closure: N^3
            coarse vs (4N)^3 fine decaying turbulence; KE(t) of fine, coarse and
            coarse+closure, wall times)
        python bench.py --cavity-study   (the 64^3 lid-driven cavity: GPU and CPU-oracle MLUPS)
@@ -334,24 +335,30 @@ def cavity_study(no_cpu=False, steps=1000):
 
 
 def storage_study(steps=2000):
-    """The sphere channel with each population storage type: step-kernel time,
-    hipGraph MLUPS and algorithmic bandwidth (FP32 collision throughout; FP64
-    for the f64 lattice; BF16/FP16 store f - 1)."""
+    """The sphere channel with each population storage type and step-kernel
+    variant (LDS-staged tiles, one cell per lane, runs of 4 cells per lane):
+    step-kernel time, hipGraph MLUPS and algorithmic bandwidth (FP32 collision
+    throughout; FP64 for the f64 lattice; BF16/FP16 store f - 1)."""
+    from asl_amd import _lib
     mapv, u = setup_fields()
     for storage, bpc in (("f32", 152.0), ("f16", 76.0), ("bf16", 76.0), ("f64", 304.0)):
         L = build_gpu(mapv, u, storage)
         counted = L.stats()["counted"]
-        L.step(200, write_macro=False, graph=True)
-        L.synchronize()
-        ms_graph, _ = L.time_steps(steps, write_macro=False, graph=True)
-        _, ms_kernel = L.time_steps(500, write_macro=False)
-        kern = ms_kernel / 500
-        byts = bpc * counted + 1.0 * int(np.prod(SIZE))
-        print(json.dumps({"study": "population storage, 256x128x128 sphere channel", "storage": storage,
-                          "bytes_per_cell_update": bpc, "kernel_ms": round(kern, 5),
-                          "graph_mlups": round(counted * steps / ms_graph / 1e3, 1),
-                          "kernel_tbps_algorithmic": round(byts / (kern / 1e3) / 1e12, 3),
-                          "frac_of_8tbps": round(byts / (kern / 1e3) / 8e12, 4)}), flush=True)
+        for variant, lds, cpl in (("tiles", 1, 0), ("one_cell_per_lane", 0, 1), ("runs_of_4", 0, 4)):
+            L.set_tuning(_lib.TUNE_LDS, lds)
+            L.set_tuning(_lib.TUNE_CPL, cpl)
+            L.step(200, write_macro=False, graph=True)
+            L.synchronize()
+            ms_graph, _ = L.time_steps(steps, write_macro=False, graph=True)
+            _, ms_kernel = L.time_steps(500, write_macro=False)
+            kern = ms_kernel / 500
+            byts = bpc * counted + 1.0 * int(np.prod(SIZE))
+            print(json.dumps({"study": "population storage, 256x128x128 sphere channel", "storage": storage,
+                              "variant": variant, "workgroup": L.tuning_value(_lib.TUNE_BLOCK),
+                              "bytes_per_cell_update": bpc, "kernel_ms": round(kern, 5),
+                              "graph_mlups": round(counted * steps / ms_graph / 1e3, 1),
+                              "kernel_tbps_algorithmic": round(byts / (kern / 1e3) / 1e12, 3),
+                              "frac_of_8tbps": round(byts / (kern / 1e3) / 8e12, 4)}), flush=True)
         del L
 
 
@@ -453,12 +460,12 @@ def batch_study(steps, warmup, group_mb=None):
                       "per_lattice_interleaved_mlups": round(cells * steps / ms_interleaved / 1e3, 1)}), flush=True)
 
 
-def closure_study(nc, tau_fine, chunk, snaps, eval_chunks, target="total"):
+def closure_study(nc, tau_fine, chunk, snaps, eval_chunks, target="total", ce_init=False):
     """Coarse N^3 with and without the learned closure against the 4x finer run
     (``asl_amd.closure.study``): one JSON line."""
     from asl_amd import closure
     r = closure.study(nc, tau_fine, chunk=chunk, snapshots=snaps, eval_chunks=eval_chunks, target=target,
-                      seed=SEED, log=lambda m: print(m, flush=True))
+                      seed=SEED, ce_init=ce_init, log=lambda m: print(m, flush=True))
     print(json.dumps(r), flush=True)
 
 
@@ -538,6 +545,9 @@ def main():
     ap.add_argument("--closure-target", choices=("total", "filtered"), default="total",
                     help="closure study: choose the output scale against the fine run's kinetic-energy decay "
                          "or against the box-filtered fine kinetic energy")
+    ap.add_argument("--ce-start", choices=("none", "coarse", "both"), default="none",
+                    help="closure study: start the coarse lattice (or both) from equilibrium plus the first "
+                         "Chapman-Enskog term of the field instead of equilibrium")
     ap.add_argument("--variant-study", type=int, default=0, metavar="N",
                     help="time plain / LBGKTurbulence / closure step kernels on a periodic N^3 lattice")
     ap.add_argument("--cavity-study", action="store_true",
@@ -565,7 +575,7 @@ def main():
         return
     if args.closure_study:
         closure_study(args.closure_study, args.tau_fine, args.chunk, args.train_snapshots, args.eval_chunks,
-                      args.closure_target)
+                      args.closure_target, {"none": False, "coarse": "coarse", "both": True}[args.ce_start])
         return
     if args.batch_study:
         batch_study(args.steps if args.steps != 5000 else 1000, args.warmup if args.warmup != 200 else 20,

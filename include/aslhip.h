@@ -369,8 +369,9 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
 #define ASLHIP_TUNE_CPL 5
 /* Lattices: 1 = the step kernel stages the populations of tiles of 256
  * consecutive cells through LDS (16-byte loads and stores of each
- * direction's slots, workgroup 256; default); 0 = direct, one cell per lane
- * (or the runs ASLHIP_TUNE_CPL asks for).  Tiles apply to the same lattices
+ * direction's slots for FP32, workgroup 256); 0 = direct, one cell per lane
+ * (or the runs ASLHIP_TUNE_CPL asks for).  Default: tiles for FP32 storage,
+ * direct for FP16 / BF16 / FP64 storage.  Tiles apply to the same lattices
  * as runs of cells when ASLHIP_TUNE_CPL is 0 or 1; results are bit for bit
  * the same either way. */
 #define ASLHIP_TUNE_LDS 6
