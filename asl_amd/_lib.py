@@ -73,6 +73,8 @@ SIGNATURES = {
     "aslhip_df_map": [_dp, ctypes.c_int, _ip, _dp, ctypes.c_double, ctypes.c_int, _vp],
     "aslhip_stl_count": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong)],
     "aslhip_stl_read": [ctypes.c_char_p, _dp, ctypes.c_longlong],
+    "aslhip_vtp_count": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong)],
+    "aslhip_vtp_read": [ctypes.c_char_p, _dp, ctypes.c_longlong],
     "aslhip_surface_distance_map": [_dp, ctypes.c_longlong, _ip, _dp, _dp, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_double, _fp],
     "aslhip_read_cell_types": [_vp, _vp],

@@ -1,10 +1,10 @@
 // aslVTKFormatReaders.h — asl::readSurface (src/readers/aslVTKFormatReaders.h:
 // 66-78) without VTK: the surface is read from an STL file (binary or ASCII)
-// and its level-set map is sampled on the GPU (aslhip_surface_distance_map).
-// The map is a float container whose ghost-inclusive grid is the sampled
-// grid (makeData, src/utilities/aslVTKCasters.cxx:562-575): map <= 0 within
-// 1.2 dx (Block variant) or 1.8 dx (dx variants) of the surface.  .vtp
-// surfaces need VTK's XML reader and are rejected.
+// or a VTK XML PolyData file (.vtp: aslhip_vtp_read) and its level-set map is
+// sampled on the GPU (aslhip_surface_distance_map).  The map is a float
+// container whose ghost-inclusive grid is the sampled grid (makeData,
+// src/utilities/aslVTKCasters.cxx:562-575): map <= 0 within 1.2 dx (Block
+// variant) or 1.8 dx (dx variants) of the surface.
 #ifndef ASLVTKFORMATREADERS_H
 #define ASLVTKFORMATREADERS_H
 

@@ -21,12 +21,12 @@ using detail::check_abi;
 std::vector<double> load_surface(const std::string &fileName) {
   const size_t dot = fileName.rfind('.');
   const std::string ext = dot == std::string::npos ? "" : fileName.substr(dot);
-  if (ext == ".vtp") errorMessage("Reader: .vtp surfaces need VTK's XML reader; convert the surface to STL");
-  if (ext != ".stl") errorMessage("Reader: file format not supported");
+  if (ext != ".stl" && ext != ".vtp") errorMessage("Reader: file format not supported");
+  const bool vtp = ext == ".vtp";
   long long n = 0;
-  check_abi(aslhip_stl_count(fileName.c_str(), &n));
+  check_abi(vtp ? aslhip_vtp_count(fileName.c_str(), &n) : aslhip_stl_count(fileName.c_str(), &n));
   std::vector<double> tri(9 * n);
-  if (n) check_abi(aslhip_stl_read(fileName.c_str(), tri.data(), n));
+  if (n) check_abi(vtp ? aslhip_vtp_read(fileName.c_str(), tri.data(), n) : aslhip_stl_read(fileName.c_str(), tri.data(), n));
   for (long long k = 0; k < 3 * n; ++k) std::swap(tri[3 * k], tri[3 * k + 2]);
   return tri;
 }

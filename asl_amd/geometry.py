@@ -101,6 +101,33 @@ def read_stl(path):
     return tri
 
 
+def read_vtp(path):
+    """Triangles of a VTK XML PolyData file (polygons as fans from their first
+    point, strips as consecutive triples), shape (n, 3, 3) float64 in the
+    file's x y z: the reading half of ``readSurface`` for ``.vtp`` files
+    (``src/readers/aslVTKFormatReaders.cxx:219-265``)."""
+    import ctypes
+
+    from . import _lib
+    p = str(path).encode()
+    n = ctypes.c_longlong(0)
+    _lib.call("aslhip_vtp_count", p, ctypes.byref(n))
+    tri = np.empty((n.value, 3, 3), dtype=np.float64)
+    if n.value:
+        _lib.call("aslhip_vtp_read", p, tri.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n.value)
+    return tri
+
+
+def read_surface_file(path):
+    """Triangles of an ``.stl`` or ``.vtp`` surface, by extension (as readSurface dispatches)."""
+    ext = str(path).rsplit(".", 1)[-1].lower() if "." in str(path) else ""
+    if ext == "stl":
+        return read_stl(path)
+    if ext == "vtp":
+        return read_vtp(path)
+    raise ValueError("readSurface: file format not supported: %s" % path)
+
+
 def surface_map(tri, shape, pos, h, cap, shift, div1, div2):
     """Distance map of triangles (n, 3, 3) in ASL axis order on the nodes
     ``pos + h * index`` of a grid of ``shape`` (on the GPU; see

@@ -270,6 +270,17 @@ int aslhip_df_map(const double *prog, int nwords, const int size[3], const doubl
 int aslhip_stl_count(const char *path, long long *ntri);
 int aslhip_stl_read(const char *path, double *tri, long long ntri);
 
+/* asl::readSurface(file, ...) for VTK XML PolyData files —
+ * src/readers/aslVTKFormatReaders.cxx:219-265 (vtkXMLPolyDataReader, then
+ * the same surfaceToData): the polygon and triangle-strip cells of every
+ * piece as triangles (a polygon as the fan from its first point, a strip as
+ * its consecutive triples), 9 doubles each (the file's x y z, points as
+ * stored: Float32 or Float64).  DataArrays in ascii, binary or appended
+ * (raw or base64) form, UInt32 or UInt64 headers, zlib-compressed or not,
+ * either byte order.  Vertex and line cells are skipped. */
+int aslhip_vtp_count(const char *path, long long *ntri);
+int aslhip_vtp_read(const char *path, double *tri, long long ntri);
+
 /* surfaceToData — src/readers/aslVTKFormatReaders.cxx:132-217 (VTK's
  * vtkImplicitModeller, capped, then rescaled): on a grid of size[3] nodes,
  * node (i, j, k) at pos + h * (i, j, k) (ASL axis order, last index fastest),

@@ -7,7 +7,7 @@
 // oracle fed the same maps (bit for bit on computation nodes), and the body
 // map with the oracle's distance map of the same triangles.
 //
-// usage: stl_body_parity file.stl [steps] [output-prefix]
+// usage: stl_body_parity file.stl|file.vtp [steps] [output-prefix]
 #include <cmath>
 #include <cstdlib>
 #include <iostream>
@@ -95,9 +95,16 @@ int main(int argc, char *argv[]) {
 
   // the body map against the oracle's map of the same triangles
   long long ntri = 0;
-  aslhip_stl_count(stl.c_str(), &ntri);
+  const bool vtp = stl.size() > 4 && stl.compare(stl.size() - 4, 4, ".vtp") == 0;
+  if (vtp)
+    aslhip_vtp_count(stl.c_str(), &ntri);
+  else
+    aslhip_stl_count(stl.c_str(), &ntri);
   std::vector<double> tri(9 * ntri);
-  aslhip_stl_read(stl.c_str(), tri.data(), ntri);
+  if (vtp)
+    aslhip_vtp_read(stl.c_str(), tri.data(), ntri);
+  else
+    aslhip_stl_read(stl.c_str(), tri.data(), ntri);
   for (long long k = 0; k < 3 * ntri; ++k) std::swap(tri[3 * k], tri[3 * k + 2]);
   const int gsize[3] = {int(S0), int(S1), int(S2)};
   double gpos[3], gh[3];

@@ -65,6 +65,27 @@ def test_cpp_stl_body_turbulence(tmp_path):
     assert (tmp_path / "out_0.vti").stat().st_size > 10000
 
 
+@pytest.mark.gpu
+def test_cpp_vtp_body_equals_stl_body(tmp_path):
+    # readSurface(file.vtp, Block) of the same triangles (appended, base64,
+    # zlib: what VTK's writer produces by default) gives the STL run's outputs
+    from test_gpu_surface_map import icosphere
+    from test_oracle_surface_map import write_stl_binary
+    from vtp import write_vtp
+    tri = np.asarray(icosphere(0.8, (2.6, 1.5, 3.0)), dtype=np.float32)
+    write_stl_binary(tmp_path / "body.stl", tri)
+    write_vtp(tmp_path / "body.vtp", [{"points": tri.reshape(-1, 3),
+                                       "polys": [[3 * k, 3 * k + 1, 3 * k + 2] for k in range(len(tri))]}],
+              form="appended", appended_encoding="base64", compress=True, block=32768)
+    outs = []
+    for ext in ("stl", "vtp"):
+        p = _run("stl_body_parity", tmp_path / ("body." + ext), 60, tmp_path / ext)
+        assert p.returncode == 0
+        assert "map bit-identical: yes" in p.stdout and ", bit-identical: yes" in p.stdout
+        outs.append((tmp_path / (ext + "_0.vti")).read_bytes())
+    assert outs[0] == outs[1]
+
+
 @pytest.mark.skipif(not os.path.isdir(REF_FLOW), reason="ASL source tree not mounted")
 @pytest.mark.parametrize("driver", ["flow", "flow2", "flow3", "flowRotatingCylinders", "locomotive",
                                     "locomotive_laminar", "locomotive_stability", "bus_wind", "compressor"])
