@@ -130,7 +130,9 @@ struct aslhip_lattice {
   VLink *vl = nullptr;
   VLinkBC *vlbc = nullptr;
   double *red_partial = nullptr, *red_out = nullptr;
-  StepArgs *d_args = nullptr;  // [STEP_F_COUNT]: step arguments for each value of StepArgs::flags
+  // [2][STEP_F_COUNT]: step arguments of even and odd steps for each value of
+  // StepArgs::flags (the two parities differ when only even steps run tiles)
+  StepArgs *d_args = nullptr;
   std::vector<StepArgs> h_args;  // host copy of d_args (empty: not uploaded yet)
   // Captured runs of plain steps (the velocity-map link fills included):
   // full chunks of ASLHIP_GRAPH_CHUNK steps and the shorter remainders of
@@ -186,7 +188,7 @@ struct aslhip_lattice {
   bool force_wide = false;  // 64-bit addressing on any size (ASLHIP_TUNE_WIDE)
   int tune_remap = 4;  // chunk runs per XCD (measured best for graph chunks; the autotuner may choose others)
   int tune_cpl = 0;    // cells per lane along z: 0 automatic, else 1, 2 or 4 (ASLHIP_TUNE_CPL)
-  int tune_lds = -1;   // LDS-staged tiles: 1 on, 0 off, -1 automatic (ASLHIP_TUNE_LDS)
+  int tune_lds = -1;   // LDS-staged tiles: 0 off, 1 every step, 2 even steps, -1 automatic (ASLHIP_TUNE_LDS)
   // face rule: per face slot * 4 + mode of the winning slice BC, and values
   int fface[6] = {0, 0, 0, 0, 0, 0};
   float fval32[6][27] = {};
@@ -609,12 +611,14 @@ bool is_wide(const aslhip_lattice *h) { return h->force_wide || h->qs_bytes * h-
 // a multiple of the run; else one cell per lane (the longest run that divides
 // nz).  Automatic: one cell per lane.
 // LDS-staged tiles apply to non-periodic lattices under 4 GiB without a
-// per-cell relaxation unless runs of 2 or 4 cells per lane are asked for.
-// Automatic: on for FP32 storage (measured fastest on the sphere channel),
-// off for FP16 / BF16 / FP64 storage (the direct kernel is faster there).
-bool effective_lds(const aslhip_lattice *h) {
-  const bool on = h->tune_lds < 0 ? h->dtype == ASLHIP_F32 : h->tune_lds != 0;
-  return on && h->tune_cpl <= 1 && !(h->periodic || is_wide(h) || h->has_closure || h->turbulence);
+// per-cell relaxation unless runs of 2 or 4 cells per lane are asked for:
+// 0 none, 1 every step, 2 even steps only (odd steps one cell per lane).
+// Automatic: every step for FP32 storage (measured fastest on the sphere
+// channel), none for FP16 / BF16 / FP64 storage (the direct kernel is
+// faster there).
+int effective_lds(const aslhip_lattice *h) {
+  const int mode = h->tune_lds < 0 ? (h->dtype == ASLHIP_F32 ? 1 : 0) : h->tune_lds;
+  return h->tune_cpl <= 1 && !(h->periodic || is_wide(h) || h->has_closure || h->turbulence) ? mode : 0;
 }
 
 int effective_cpl(const aslhip_lattice *h) {
@@ -626,16 +630,17 @@ int effective_cpl(const aslhip_lattice *h) {
 }
 
 // Workgroup size the step kernel runs with (runs of cells: at most
-// STEP_RUN_BLOCK; LDS-staged tiles: STEP_TILE).
+// STEP_RUN_BLOCK; LDS-staged tiles on every step: STEP_TILE; tiles on even
+// steps only: the odd steps' workgroup).
 int step_workgroup(const aslhip_lattice *h) {
-  if (effective_lds(h)) return STEP_TILE;
+  if (effective_lds(h) == 1) return STEP_TILE;
   return effective_cpl(h) > 1 && h->tune_block > STEP_RUN_BLOCK ? STEP_RUN_BLOCK : h->tune_block;
 }
 
 // Kernel arguments of one step of h (flags filled in by the caller).  With
-// block and cpl 0 the lattice's own tuning (LDS-staged tiles included), else
-// one cell per lane in workgroups of that size.
-StepArgs step_args(const aslhip_lattice *h, int block = 0, int cpl = 0) {
+// block and cpl 0 the lattice's own tuning for a step of that parity (LDS-
+// staged tiles included), else one cell per lane in workgroups of that size.
+StepArgs step_args(const aslhip_lattice *h, int block = 0, int cpl = 0, bool odd = false) {
   StepArgs a;
   std::memset(&a, 0, sizeof a);  // compared bytewise by upload_args
   for (int k = 0; k < 27; ++k) a.fd[k] = k < h->Q ? h->f + k * h->qs_bytes : nullptr;
@@ -656,7 +661,8 @@ StepArgs step_args(const aslhip_lattice *h, int block = 0, int cpl = 0) {
   a.cpl = cpl > 0 ? cpl : effective_cpl(h);
   a.block = block > 0 ? block : h->tune_block;
   if (a.cpl > 1 && a.block > STEP_RUN_BLOCK) a.block = STEP_RUN_BLOCK;
-  a.tile = block <= 0 && cpl <= 0 && effective_lds(h) ? 1 : 0;
+  const int lds = effective_lds(h);
+  a.tile = block <= 0 && cpl <= 0 && (lds == 1 || (lds == 2 && !odd)) ? 1 : 0;
   if (a.tile) a.block = STEP_TILE;
 
   a.nzv = h->n[2] / a.cpl;
@@ -705,29 +711,37 @@ int prepare(aslhip_lattice *h, bool allow_open) {
 // Brings the device copies of the step arguments (one per flags value) up to
 // date; they change with the BC tables, viscosity, compressibility and tuning.
 int upload_args(aslhip_lattice *h) {
-  const StepArgs base = step_args(h);
+  const StepArgs base[2] = {step_args(h, 0, 0, false), step_args(h, 0, 0, true)};
   if (!h->h_args.empty()) {
-    if (std::memcmp(&base, &h->h_args[base.flags], sizeof base) == 0) return 0;
+    if (std::memcmp(&base[0], &h->h_args[base[0].flags], sizeof(StepArgs)) == 0 &&
+        std::memcmp(&base[1], &h->h_args[STEP_F_COUNT + base[1].flags], sizeof(StepArgs)) == 0)
+      return 0;
     // the previous upload reads h_args: let it finish before overwriting
     HIPCHK(hipStreamSynchronize(h->stream));
   }
-  h->h_args.assign(STEP_F_COUNT, base);
-  for (unsigned fl = 0; fl < STEP_F_COUNT; ++fl) h->h_args[fl].flags = fl;
+  h->h_args.resize(2 * STEP_F_COUNT);
+  for (int p = 0; p < 2; ++p)
+    for (unsigned fl = 0; fl < STEP_F_COUNT; ++fl) {
+      h->h_args[p * STEP_F_COUNT + fl] = base[p];
+      h->h_args[p * STEP_F_COUNT + fl].flags = fl;
+    }
   // stream-ordered: steps queued before still see the previous arguments
-  HIPCHK(hipMemcpyAsync(h->d_args, h->h_args.data(), STEP_F_COUNT * sizeof(StepArgs), hipMemcpyHostToDevice,
+  HIPCHK(hipMemcpyAsync(h->d_args, h->h_args.data(), 2 * STEP_F_COUNT * sizeof(StepArgs), hipMemcpyHostToDevice,
                         h->stream));
   return 0;
 }
 
 int launch_one(aslhip_lattice *h, unsigned fl, std::vector<hipEvent_t> *ev, int s) {
   StepLaunch a;
-  a.nblocks = h->h_args[0].nblocks;
-  a.block = h->h_args[0].block;
-  a.args = h->d_args + fl;
+  const int p0 = h->next_odd ? STEP_F_COUNT : 0;  // this parity's argument table
+  const StepArgs &A = h->h_args[p0];
+  a.nblocks = A.nblocks;
+  a.block = A.block;
+  a.args = h->d_args + p0 + fl;
   // LBGKTurbulence takes precedence over a closure in the collision
   a.variant = h->turbulence ? 1 : (h->has_closure ? (h->closure_mfma ? 2 : 1) : 0);
-  a.cpl = h->h_args[0].cpl;
-  a.tile = h->h_args[0].tile;
+  a.cpl = A.cpl;
+  a.tile = A.tile;
   if (ev) HIPCHK(hipEventRecord((*ev)[2 * s], h->stream));
   HIPCHK(launch_step(h->Q, h->dtype, h->next_odd, h->periodic, is_wide(h), a, h->stream));
   if (ev) HIPCHK(hipEventRecord((*ev)[2 * s + 1], h->stream));
@@ -984,7 +998,7 @@ int aslhip_lattice_create(aslhip_lattice **out, int nx, int ny, int nz, int q, i
   if (e == hipSuccess) e = hipMalloc(&h->mode, 256);
   if (e == hipSuccess) e = hipMalloc(&h->val, 256 * (size_t)q * h->celem);
   if (e == hipSuccess) e = hipMalloc(&h->vlbc, 256 * sizeof(VLinkBC));
-  if (e == hipSuccess) e = hipMalloc(&h->d_args, STEP_F_COUNT * sizeof(StepArgs));
+  if (e == hipSuccess) e = hipMalloc(&h->d_args, 2 * STEP_F_COUNT * sizeof(StepArgs));
   if (e == hipSuccess) e = hipMalloc(&h->red_partial, (size_t)reduce_blocks((int)len) * 5 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->red_out, 5 * sizeof(double));
   if (e != hipSuccess) {
@@ -1575,7 +1589,7 @@ int aslhip_set_tuning(aslhip_lattice *h, int knob, int value) {
     return 0;
   }
   if (knob == ASLHIP_TUNE_LDS) {
-    if (value != 0 && value != 1) return fail(ASLHIP_EINVAL, "ASL ERROR: ASLHIP_TUNE_LDS takes 0 or 1");
+    if (value < 0 || value > 2) return fail(ASLHIP_EINVAL, "ASL ERROR: ASLHIP_TUNE_LDS takes 0, 1 or 2");
     if (int r = flush_deferred(h)) return r;
     if (value != h->tune_lds) drop_graphs(h);  // captured launches name the variant and grid
     h->tune_lds = value;
@@ -1592,7 +1606,7 @@ int aslhip_tuning_value(aslhip_lattice *h, int knob, int *value) {
     case ASLHIP_TUNE_BLOCK: *value = step_workgroup(h); return 0;
     case ASLHIP_TUNE_WIDE: *value = is_wide(h) ? 1 : 0; return 0;
     case ASLHIP_TUNE_CPL: *value = effective_cpl(h); return 0;
-    case ASLHIP_TUNE_LDS: *value = effective_lds(h) ? 1 : 0; return 0;
+    case ASLHIP_TUNE_LDS: *value = effective_lds(h); return 0;
     default: return fail(ASLHIP_EINVAL, "ASL ERROR: unknown tuning knob %d", knob);
   }
 }
@@ -1603,7 +1617,7 @@ int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_re
   nsteps += nsteps & 1;
   const int remaps[6] = {0, 2, 4, 8, 16, 64};
   double best = 1e300;
-  int bb = h->tune_block, br = h->tune_remap, bc = h->tune_cpl, bl = effective_lds(h) ? 1 : 0;
+  int bb = h->tune_block, br = h->tune_remap, bc = h->tune_cpl, bl = effective_lds(h);
   auto trial = [&](int lds, int cpl, int remap, int blk) -> int {
     if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_LDS, lds)) return r;
     if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_CPL, cpl)) return r;
@@ -1620,12 +1634,17 @@ int aslhip_autotune(aslhip_lattice *h, int nsteps, int *best_block, int *best_re
     }
     return 0;
   };
-  // LDS-staged tiles (fixed workgroup) where the lattice can run them
+  // LDS-staged tiles where the lattice can run them: on every step (fixed
+  // workgroup), on even steps (workgroup of the odd steps' direct kernel)
   if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_CPL, 0)) return r;
   if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_LDS, 1)) return r;
-  if (effective_lds(h))
+  if (effective_lds(h)) {
     for (int remap : remaps)
       if (int r = trial(1, 0, remap, h->tune_block)) return r;
+    for (int remap : remaps)
+      for (int blk : {256, 512, 1024})
+        if (int r = trial(2, 0, remap, blk)) return r;
+  }
   // direct: the distinct cells-per-lane values this lattice can run
   if (int r = aslhip_set_tuning(h, ASLHIP_TUNE_LDS, 0)) return r;
   std::vector<int> cpls;

@@ -344,7 +344,8 @@ def storage_study(steps=2000):
     for storage, bpc in (("f32", 152.0), ("f16", 76.0), ("bf16", 76.0), ("f64", 304.0)):
         L = build_gpu(mapv, u, storage)
         counted = L.stats()["counted"]
-        for variant, lds, cpl in (("tiles", 1, 0), ("one_cell_per_lane", 0, 1), ("runs_of_4", 0, 4)):
+        for variant, lds, cpl in (("tiles", 1, 0), ("tiles_even_steps", 2, 0), ("one_cell_per_lane", 0, 1),
+                                  ("runs_of_4", 0, 4)):
             L.set_tuning(_lib.TUNE_LDS, lds)
             L.set_tuning(_lib.TUNE_CPL, cpl)
             L.step(200, write_macro=False, graph=True)
@@ -371,14 +372,15 @@ def launch_study():
     L.step(20, write_macro=False)
     counted = L.stats()["counted"]
     byts = 152.0 * counted + 1.0 * int(np.prod(SIZE))
-    # cells per lane 1, 2, 4, then "lds": LDS-staged tiles of 256 cells
-    for cpl in (1, 2, 4, "lds"):
+    # cells per lane 1, 2, 4 (direct), then LDS-staged tiles of 256 cells on
+    # every step ("lds") and on even steps only ("lds_even", odd steps direct)
+    for cpl in (1, 2, 4, "lds", "lds_even"):
         for remap in (0, 4, 16, 64):
             for block in (256, 512, 1024):
-                if cpl != 1 and block > 512 or cpl == "lds" and block != 256:
+                if cpl in (2, 4) and block > 512 or cpl == "lds" and block != 256:
                     continue
-                L.set_tuning(_lib.TUNE_LDS, 1 if cpl == "lds" else 0)
-                L.set_tuning(_lib.TUNE_CPL, 1 if cpl == "lds" else cpl)
+                L.set_tuning(_lib.TUNE_LDS, {"lds": 1, "lds_even": 2}.get(cpl, 0))
+                L.set_tuning(_lib.TUNE_CPL, cpl if cpl in (1, 2, 4) else 0)
                 L.set_tuning(_lib.TUNE_XCD_REMAP, remap)
                 L.set_tuning(_lib.TUNE_BLOCK, block)
                 L.time_steps(20, write_macro=False)
@@ -524,8 +526,8 @@ def main():
     ap.add_argument("--cpl", type=int, default=-1, metavar="C",
                     help="cells per lane of the step kernel (1, 2, 4; default: the library's choice)")
     ap.add_argument("--lds", type=int, default=-1, metavar="B",
-                    help="1: the step kernel stages tiles of 256 cells through LDS; 0: direct "
-                         "(default: the library's choice)")
+                    help="1: the step kernel stages tiles of 256 cells through LDS on every step; 2: on even "
+                         "steps only; 0: direct (default: the library's choice)")
     ap.add_argument("--launch-study", action="store_true",
                     help="step-kernel duration on the sphere channel for every cells-per-lane x XCD run x workgroup")
     ap.add_argument("--autotune", action="store_true",

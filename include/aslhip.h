@@ -380,25 +380,28 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
 #define ASLHIP_TUNE_CPL 5
 /* Lattices: 1 = the step kernel stages the populations of tiles of 256
  * consecutive cells through LDS (16-byte loads and stores of each
- * direction's slots for FP32, workgroup 256); 0 = direct, one cell per lane
- * (or the runs ASLHIP_TUNE_CPL asks for).  Default: tiles for FP32 storage,
- * direct for FP16 / BF16 / FP64 storage.  Tiles apply to the same lattices
- * as runs of cells when ASLHIP_TUNE_CPL is 0 or 1; results are bit for bit
- * the same either way. */
+ * direction's slots for FP32, workgroup 256); 2 = tiles on even steps only,
+ * odd steps direct; 0 = direct, one cell per lane (or the runs
+ * ASLHIP_TUNE_CPL asks for).  Default: 1 for FP32 storage, 0 for FP16 /
+ * BF16 / FP64 storage.  Tiles apply to the same lattices as runs of cells
+ * when ASLHIP_TUNE_CPL is 0 or 1; results are bit for bit the same for
+ * every value. */
 #define ASLHIP_TUNE_LDS 6
 int aslhip_set_tuning(aslhip_lattice *h, int knob, int value);
 /* The value a tuning knob has now (ASLHIP_TUNE_CPL: the cells per lane the
- * step kernel runs with; ASLHIP_TUNE_LDS: 1 if it runs LDS-staged tiles). */
+ * step kernel runs with; ASLHIP_TUNE_LDS: 0, 1 or 2 as it runs LDS-staged
+ * tiles on no step, every step or even steps). */
 int aslhip_tuning_value(aslhip_lattice *h, int knob, int *value);
 /* The workgroup size the step kernel runs with (runs of 2 or 4 cells per lane:
- * at most 512; LDS-staged tiles: 256) and the XCD run length (either may be
- * NULL). */
+ * at most 512; LDS-staged tiles on every step: 256; tiles on even steps only:
+ * the odd steps' workgroup) and the XCD run length (either may be NULL). */
 int aslhip_get_tuning(aslhip_lattice *h, int *block, int *remap);
 
 /* The autotuner: times nsteps (rounded up to even) real steps of the lattice
- * for every candidate setting (LDS-staged tiles x XCD runs 0/2/4/8/16/64
- * where the lattice can run them; direct with cells per lane 1/2/4 where it
- * can run them x XCD runs 0/2/4/8/16/64 x workgroup 256/512/1024) with HIP
+ * for every candidate setting (where the lattice can run them: LDS-staged
+ * tiles on every step x XCD runs 0/2/4/8/16/64, tiles on even steps x XCD
+ * runs x odd-step workgroup 256/512/1024; direct with cells per lane 1/2/4
+ * where it can run them x XCD runs x workgroup 256/512/1024) with HIP
  * events, keeps the fastest and returns its workgroup size and XCD run
  * length (either pointer may be NULL; the variant: aslhip_tuning_value).
  * The trial steps advance the simulation like any other steps. */

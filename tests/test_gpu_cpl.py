@@ -31,10 +31,13 @@ def sphere(S, c, r):
 
 
 def build(size, q, dtype, cpl, case):
-    """cpl: cells per lane (1: the direct kernel), or "lds" for LDS-staged tiles."""
+    """cpl: cells per lane (1: the direct kernel), "lds" for LDS-staged tiles
+    on every step, "lds_even" for tiles on even steps only."""
     L = asl_amd.Lattice(size, q, dtype=dtype, nu=0.05)
     if cpl == "lds":
         L.set_tuning(_lib.TUNE_LDS, 1)
+    elif cpl == "lds_even":
+        L.set_tuning(_lib.TUNE_LDS, 2)
     else:
         L.set_tuning(_lib.TUNE_LDS, 0)
         L.set_tuning(_lib.TUNE_CPL, cpl)
@@ -84,10 +87,12 @@ def test_runs_equal_single_cells(case, q):
     ref = build(size, q, "f32", 1, case)
     ref.step(41)
     want = state(ref)
-    for cpl in (2, 4, "lds"):
+    for cpl in (2, 4, "lds", "lds_even"):
         L = build(size, q, "f32", cpl, case)
         if cpl == "lds":
             assert L.tuning_value(_lib.TUNE_LDS) == 1 and L.tuning()[0] == 256
+        elif cpl == "lds_even":
+            assert L.tuning_value(_lib.TUNE_LDS) == 2 and L.tuning()[0] == 512
         else:
             assert L.tuning_value(_lib.TUNE_CPL) == cpl
         L.step(41)
@@ -101,9 +106,10 @@ def test_tiles_across_rows_equal_single_cells(case, q):
     size = (20, 23, 30)
     ref = build(size, q, "f32", 1, case)
     ref.step(23)
-    L = build(size, q, "f32", "lds", case)
-    L.step(23)
-    assert same(state(L), state(ref)), (case, q)
+    for v in ("lds", "lds_even"):
+        L = build(size, q, "f32", v, case)
+        L.step(23)
+        assert same(state(L), state(ref)), (case, q, v)
 
 
 @pytest.mark.parametrize("dtype", ["f64", "bf16", "f16"])
@@ -133,16 +139,18 @@ def test_runs_under_graph_replay_and_tuning_changes():
     L.set_tuning(_lib.TUNE_LDS, 1)
     assert L.tuning_value(_lib.TUNE_LDS) == 1
     L.step(131, graph=True)
+    L.set_tuning(_lib.TUNE_LDS, 2)
+    L.step(77, graph=True)
     L.set_tuning(_lib.TUNE_LDS, 0)
     L.step(20, graph=True)
-    ref.step(151)
+    ref.step(228)
     assert same(state(L), state(ref))
 
 
 def test_run_length_follows_nz():
     for nz, want in ((12, 4), (10, 2), (9, 1)):
         L = asl_amd.Lattice((6, 5, nz), 19, nu=0.1)
-        # the default: LDS-staged tiles, one cell per lane
+        # the default for FP32: LDS-staged tiles on every step, one cell per lane
         assert L.tuning_value(_lib.TUNE_CPL) == 1 and L.tuning_value(_lib.TUNE_LDS) == 1
         L.set_tuning(_lib.TUNE_CPL, 4)
         assert L.tuning_value(_lib.TUNE_CPL) == want
@@ -156,12 +164,18 @@ def test_run_length_follows_nz():
     L.set_tuning(_lib.TUNE_LDS, 1)
     assert L.tuning_value(_lib.TUNE_LDS) == 0  # periodic wrap: direct
     with pytest.raises(asl_amd.ASLError):
-        L.set_tuning(_lib.TUNE_LDS, 2)
+        L.set_tuning(_lib.TUNE_LDS, -1)
     L = asl_amd.Lattice((6, 5, 12), 19, nu=0.1)
     L.set_tuning(_lib.TUNE_CPL, 4)  # runs take precedence over tiles
     assert L.tuning_value(_lib.TUNE_LDS) == 0 and L.tuning_value(_lib.TUNE_CPL) == 4
     L.set_tuning(_lib.TUNE_CPL, 0)
     assert L.tuning_value(_lib.TUNE_LDS) == 1 and L.tuning_value(_lib.TUNE_CPL) == 1
-    assert L.tuning()[0] == 256
+    assert L.tuning() == (256, 4)
+    L.set_tuning(_lib.TUNE_LDS, 2)
+    assert L.tuning() == (512, 4)
     L.set_tuning(_lib.TUNE_LDS, 0)
     assert L.tuning() == (512, 4)
+    with pytest.raises(asl_amd.ASLError):
+        L.set_tuning(_lib.TUNE_LDS, 3)
+    for dtype in ("f16", "bf16", "f64"):  # direct by default for these storage types
+        assert asl_amd.Lattice((6, 5, 12), 19, dtype=dtype, nu=0.1).tuning_value(_lib.TUNE_LDS) == 0

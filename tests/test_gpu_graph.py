@@ -87,7 +87,8 @@ def test_launch_tuning_and_autotune_keep_results():
     """Workgroup size and chunk order change the launch only, never the result."""
     ref, _ = cavity()
     ref.step(120)
-    for lds, block, remap in ((0, 256, 0), (0, 512, 16), (0, 64, 3), (0, 1024, 4096), (1, 512, 0), (1, 512, 4096)):
+    for lds, block, remap in ((0, 256, 0), (0, 512, 16), (0, 64, 3), (0, 1024, 4096), (1, 512, 0), (1, 512, 4096),
+                              (2, 1024, 0), (2, 64, 16)):
         L, _ = cavity()
         L.set_tuning(_lib.TUNE_LDS, lds)
         L.set_tuning(_lib.TUNE_BLOCK, block)
@@ -96,11 +97,11 @@ def test_launch_tuning_and_autotune_keep_results():
         L.step(50)
         same(L, ref)
     L, _ = cavity()
-    assert L.tuning() == (256, 4)  # the defaults: LDS-staged tiles of 256 cells
-    # candidates: tiles x 6 XCD runs; direct with cells per lane 1 and 2
-    # (nz = 22 is not a multiple of 4) x 6 XCD runs x 3 workgroup sizes; 10
-    # real steps each
-    ncand = 6 + 2 * 6 * 3
+    assert L.tuning() == (256, 4)  # the defaults: LDS-staged tiles of 256 cells on every step
+    # candidates: tiles on every step x 6 XCD runs; tiles on even steps x 6 XCD
+    # runs x 3 odd-step workgroups; direct with cells per lane 1 and 2 (nz = 22
+    # is not a multiple of 4) x 6 XCD runs x 3 workgroup sizes; 10 real steps each
+    ncand = 6 + 6 * 3 + 2 * 6 * 3
     ref.step(10 * ncand)  # ref: 120 + 10 ncand steps; L: 10 ncand + 120
     best = L.autotune(10)
     assert best[0] in (256, 512, 1024) and best[1] in (0, 2, 4, 8, 16, 64)
