@@ -1231,6 +1231,14 @@ __global__ __launch_bounds__(STEP_BLOCK) void k_step_batch(const StepArgs *desc,
   step_body<Q, S, ODD, PER, WIDE>(desc[e.x], static_cast<int>(e.y));
 }
 
+// The same with LDS-staged tiles (entries are tiles of STEP_TILE cells).
+template <int Q, class S, bool ODD>
+__global__ __launch_bounds__(STEP_TILE) void k_step_batch_tile(const StepArgs *desc, const uint2 *ent) {
+  const int b = static_cast<int>(blockIdx.x);
+  const uint2 e = ent[ODD ? xcd_reverse(b, static_cast<int>(gridDim.x)) : b];
+  step_body_tile<Q, S, ODD>(desc[e.x], static_cast<int>(e.y));
+}
+
 // Closure features of every interior cell of a periodic D3Q19 lattice: the
 // populations the next step would collide (AA addressing of that step's
 // parity, periodic wrap), their moments as collide() forms them, and the 7
@@ -1515,6 +1523,11 @@ hipError_t step_w(bool wide, const StepLaunch &a, hipStream_t s) {
 template <int Q, class S, bool ODD, bool PER>
 hipError_t batch_w(bool wide, const BatchLaunch &b, hipStream_t s) {
   const dim3 grid(b.nblocks), block(b.block);
+  if (b.tile) {
+    if (PER || wide || b.block != STEP_TILE) return hipErrorInvalidValue;  // the host sends only these
+    if constexpr (!PER) hipLaunchKernelGGL((k_step_batch_tile<Q, S, ODD>), grid, block, 0, s, b.desc, b.ent);
+    return hipGetLastError();
+  }
   if (wide)
     hipLaunchKernelGGL((k_step_batch<Q, S, ODD, PER, true>), grid, block, 0, s, b.desc, b.ent);
   else

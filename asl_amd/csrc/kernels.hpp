@@ -83,6 +83,7 @@ struct BatchLaunch {
   const uint2 *ent;  // per workgroup: lattice, chunk
   int nblocks;
   int block;
+  int tile;  // 1: LDS-staged tiles (block = STEP_TILE; non-periodic, under 4 GiB)
 };
 
 // Boundary-link kernel entries (velocity map BCs).

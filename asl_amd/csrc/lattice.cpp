@@ -1721,7 +1721,8 @@ struct aslhip_batch {
   StepArgs *desc = nullptr;  // [2][n]: plain steps, then steps writing rho/u
   uint2 *ent = nullptr;      // per workgroup: lattice, chunk
   int nblocks = 0;
-  int block = STEP_BLOCK;    // workgroup size of all lattices of the batch
+  int block = STEP_BLOCK;    // workgroup size of all lattices of the batch (direct kernel)
+  int tune_lds = -1;         // LDS-staged tiles: 1 on, 0 off, -1 automatic (FP32 storage)
   // Graph chunks go group by group: each group of consecutive lattices (at
   // most group_mb MiB of populations; 0: one group) takes all the chunk's
   // steps before the next starts, so a group that fits the 256 MiB Infinity
@@ -1733,6 +1734,21 @@ struct aslhip_batch {
 };
 
 namespace {
+
+// Tiles for the batch's steps: non-periodic batches under 4 GiB per lattice;
+// automatic for FP32 storage, as for single lattices.
+bool batch_tile(const aslhip_batch *b) {
+  const bool on = b->tune_lds < 0 ? b->dtype == ASLHIP_F32 : b->tune_lds != 0;
+  return on && !b->periodic && !b->wide;
+}
+
+// One lattice's step arguments in the batch's launch shape.
+StepArgs batch_args(const aslhip_batch *b, const aslhip_lattice *h) {
+  const bool tile = batch_tile(b);
+  StepArgs a = step_args(h, tile ? STEP_TILE : b->block, 1);
+  a.tile = tile ? 1 : 0;
+  return a;
+}
 
 void batch_free(aslhip_batch *b) {
   for (auto &g : b->graph)
@@ -1750,7 +1766,8 @@ int batch_launch(aslhip_batch *b, bool odd, bool macro, int e0, int e1) {
   L.desc = b->desc + (macro ? b->lat.size() : 0);
   L.ent = b->ent + e0;
   L.nblocks = e1 - e0;
-  L.block = b->block;
+  L.tile = batch_tile(b) ? 1 : 0;
+  L.block = L.tile ? STEP_TILE : b->block;
   HIPCHK(launch_step_batch(b->Q, b->dtype, odd, b->periodic, b->wide, L, b->stream));
   return 0;
 }
@@ -1771,7 +1788,7 @@ int batch_tables(aslhip_batch *b) {
       gbytes = 0.;
     }
     gbytes += bytes;
-    const StepArgs a = step_args(b->lat[l], b->block, 1);
+    const StepArgs a = batch_args(b, b->lat[l]);
     for (int c = 0; c < a.nblocks; ++c) ent.push_back(make_uint2(static_cast<unsigned>(l), static_cast<unsigned>(c)));
   }
   b->groups.emplace_back(static_cast<int>(gfirst), static_cast<int>(ent.size()));
@@ -1844,7 +1861,7 @@ int batch_step_impl(aslhip_batch *b, int nsteps, unsigned flags) {
       return fail(ASLHIP_EINVAL, "ASL ERROR: per-cell relaxation (closure, turbulence) is not supported in batches");
     if (h->next_odd != odd) return fail(ASLHIP_ESTATE, "ASL ERROR: batch lattices are at different step parities");
     if (int r = order_after_consumer(h)) return r;
-    desc[l] = step_args(h, b->block, 1);
+    desc[l] = batch_args(b, h);
     desc[n + l] = desc[l];
     if (h->comp_rho) desc[n + l].flags |= STEP_F_RHO;
     if (h->comp_vel) desc[n + l].flags |= STEP_F_VEL;
@@ -1918,8 +1935,14 @@ int aslhip_batch_set_tuning(aslhip_batch *b, int knob, int value) {
     b->group_mb = value;
     return batch_tables(b);
   }
+  if (knob == ASLHIP_TUNE_LDS) {
+    if (value != 0 && value != 1) return fail(ASLHIP_EINVAL, "ASL ERROR: batches take ASLHIP_TUNE_LDS 0 or 1");
+    if (value == b->tune_lds) return 0;
+    b->tune_lds = value;
+    return batch_tables(b);
+  }
   if (knob != ASLHIP_TUNE_BLOCK)
-    return fail(ASLHIP_EINVAL, "ASL ERROR: batches tune the workgroup size and the group size only");
+    return fail(ASLHIP_EINVAL, "ASL ERROR: batches tune the workgroup size, the tiles and the group size only");
   if (value < 64 || value > STEP_BLOCK || value % 64)
     return fail(ASLHIP_EINVAL, "ASL ERROR: workgroup size %d is not a multiple of 64 in 64..%d", value, STEP_BLOCK);
   if (value == b->block) return 0;
@@ -1932,18 +1955,26 @@ int aslhip_batch_autotune(aslhip_batch *b, int nsteps, int *best_block) {
   if (nsteps < 1) return fail(ASLHIP_EINVAL, "ASL ERROR: autotune needs at least one step per candidate");
   nsteps += nsteps & 1;
   double best = 1e300;
-  int bb = b->block;
-  for (int blk : {256, 512, 1024}) {
+  int bb = b->block, bl = batch_tile(b) ? 1 : 0;
+  auto trial = [&](int lds, int blk) -> int {
+    if (int r = aslhip_batch_set_tuning(b, ASLHIP_TUNE_LDS, lds)) return r;
     if (int r = aslhip_batch_set_tuning(b, ASLHIP_TUNE_BLOCK, blk)) return r;
     double ms = 0.;
     if (int r = aslhip_batch_time_steps(b, nsteps, 0u, &ms)) return r;
     if (ms < best) {
       best = ms;
       bb = blk;
+      bl = lds;
     }
-  }
+    return 0;
+  };
+  if (!b->periodic && !b->wide)
+    if (int r = trial(1, b->block)) return r;  // tiles: fixed workgroup
+  for (int blk : {256, 512, 1024})
+    if (int r = trial(0, blk)) return r;
+  if (int r = aslhip_batch_set_tuning(b, ASLHIP_TUNE_LDS, bl)) return r;
   if (int r = aslhip_batch_set_tuning(b, ASLHIP_TUNE_BLOCK, bb)) return r;
-  if (best_block) *best_block = bb;
+  if (best_block) *best_block = bl ? STEP_TILE : bb;
   return 0;
 }
 

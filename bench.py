@@ -433,6 +433,14 @@ def batch_study(steps, warmup, group_mb=None):
         from asl_amd import _lib
         B.set_tuning(_lib.TUNE_GROUP_MB, group_mb)
     ms_batch = B.time_steps(steps, graph=True)
+    # the two step kernels for the whole batch (LDS-staged tiles; direct, one
+    # cell per lane in the workgroup the autotuner last timed for it)
+    from asl_amd import _lib
+    kernels = {}
+    for name, lds in (("tiles", 1), ("direct", 0)):
+        B.set_tuning(_lib.TUNE_LDS, lds)
+        B.step(100, write_macro=False, graph=True)  # captures this variant's chunk
+        kernels[name] = round(cells * steps / B.time_steps(steps, graph=True) / 1e3, 1)
     solo = [batch_cavity(n, t) for n, t in specs]
     for L in solo:
         L.step(max(warmup, 1), write_macro=False)
@@ -456,6 +464,7 @@ def batch_study(steps, warmup, group_mb=None):
     print(json.dumps({"study": "batched lattices", "lattices": len(specs), "cells": cells,
                       "sizes": [n for n, _ in specs], "steps": steps,
                       "batch_graph_mlups": round(cells * steps / ms_batch / 1e3, 1), "batch_block": block,
+                      "batch_graph_mlups_by_kernel": kernels,
                       "batch_group_mb": group_mb,
                       "batch_gbps": round(152.0 * cells * steps / ms_batch / 1e6, 1),
                       "per_lattice_streams_mlups": round(cells * steps / ms_solo / 1e3, 1),

@@ -475,10 +475,12 @@ int aslhip_batch_destroy(aslhip_batch *b);
 int aslhip_batch_step(aslhip_batch *b, int nsteps, unsigned flags);
 /* As aslhip_batch_step, timed with HIP events on the batch's stream. */
 int aslhip_batch_time_steps(aslhip_batch *b, int nsteps, unsigned flags, double *ms_total);
-/* Workgroup size (ASLHIP_TUNE_BLOCK) or group size (ASLHIP_TUNE_GROUP_MB)
- * of the batch's launches; the autotuners choose the workgroup size
- * (256/512/1024, timing nsteps real eager steps each) and the group size
- * (0, 64 .. 384 MiB, timing `chunks` real graph chunks each). */
+/* Workgroup size (ASLHIP_TUNE_BLOCK), LDS-staged tiles (ASLHIP_TUNE_LDS 1
+ * or 0; default: tiles for FP32 storage on non-periodic batches) or group
+ * size (ASLHIP_TUNE_GROUP_MB) of the batch's launches; the autotuners choose
+ * between tiles and the direct kernel's workgroup size (256/512/1024,
+ * timing nsteps real eager steps each; returns 256 for tiles) and the group
+ * size (0, 64 .. 384 MiB, timing `chunks` real graph chunks each). */
 int aslhip_batch_set_tuning(aslhip_batch *b, int knob, int value);
 int aslhip_batch_autotune(aslhip_batch *b, int nsteps, int *best_block);
 int aslhip_batch_autotune_groups(aslhip_batch *b, int chunks, int *best_mb);

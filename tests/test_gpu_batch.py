@@ -86,10 +86,13 @@ def test_batch_workgroup_size_keeps_results():
         L.step(130)
     members = [cavity(n, t) for n, t in SPECS]
     B = asl_amd.Batch(members)
+    B.set_tuning(_lib.TUNE_LDS, 0)  # the direct kernel (FP32 batches default to LDS tiles)
     B.set_tuning(_lib.TUNE_BLOCK, 256)
-    B.step(60, graph=True)
-    assert B.autotune(5) in (256, 512, 1024)  # 3 x 6 real steps
-    B.step(130 - 60 - 18, graph=True)
+    B.step(30, graph=True)
+    B.set_tuning(_lib.TUNE_LDS, 1)
+    B.step(30, graph=True)
+    assert B.autotune(5) in (256, 512, 1024)  # (tiles + 3 workgroup sizes) x 6 real steps
+    B.step(130 - 60 - 24, graph=True)
     for L, R in zip(members, solo):
         assert np.array_equal(interior(L.f()), interior(R.f()))
 
