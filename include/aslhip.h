@@ -357,8 +357,8 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
 
 /* Launch tuning of the step kernel (no effect on results).
  *   ASLHIP_TUNE_XCD_REMAP: g > 1 = each of the 8 XCDs takes runs of g
- *   consecutive chunks (neighbouring chunks share one L2; default 4); 0 or
- *   1 = the hardware round-robin order.
+ *   consecutive chunks (neighbouring chunks share one L2; default 4, 64 for
+ *   FP16 / BF16 storage); 0 or 1 = the hardware round-robin order.
  *   ASLHIP_TUNE_BLOCK: workgroup size, a multiple of 64 up to 1024 (default
  *   512); a workgroup takes that many consecutive cells of a plane. */
 #define ASLHIP_TUNE_XCD_REMAP 1

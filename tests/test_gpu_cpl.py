@@ -178,4 +178,6 @@ def test_run_length_follows_nz():
     with pytest.raises(asl_amd.ASLError):
         L.set_tuning(_lib.TUNE_LDS, 3)
     for dtype in ("f16", "bf16", "f64"):  # direct by default for these storage types
-        assert asl_amd.Lattice((6, 5, 12), 19, dtype=dtype, nu=0.1).tuning_value(_lib.TUNE_LDS) == 0
+        L = asl_amd.Lattice((6, 5, 12), 19, dtype=dtype, nu=0.1)
+        assert L.tuning_value(_lib.TUNE_LDS) == 0
+        assert L.tuning() == (512, 4 if dtype == "f64" else 64)  # 16-bit storage: XCD runs of 64
