@@ -613,11 +613,15 @@ bool is_wide(const aslhip_lattice *h) { return h->force_wide || h->qs_bytes * h-
 // LDS-staged tiles apply to non-periodic lattices under 4 GiB without a
 // per-cell relaxation unless runs of 2 or 4 cells per lane are asked for:
 // 0 none, 1 every step, 2 even steps only (odd steps one cell per lane).
-// Automatic: every step for FP32 storage (measured fastest on the sphere
-// channel), none for FP16 / BF16 / FP64 storage (the direct kernel is
+// Automatic: every step for FP32 storage whose populations outgrow the
+// 256 MiB Infinity Cache (the sphere channel, 331 MB: 0.1225 against 0.128
+// ms per step); none for lattices the cache holds (64^3 cavity, 22 MB: 15 us
+// against 11 us) and for FP16 / BF16 / FP64 storage (the direct kernel is
 // faster there).
+constexpr long long TILE_MIN_BYTES = 256LL << 20;
 int effective_lds(const aslhip_lattice *h) {
-  const int mode = h->tune_lds < 0 ? (h->dtype == ASLHIP_F32 ? 1 : 0) : h->tune_lds;
+  const bool big = h->qs_bytes * h->Q > TILE_MIN_BYTES;
+  const int mode = h->tune_lds < 0 ? (h->dtype == ASLHIP_F32 && big ? 1 : 0) : h->tune_lds;
   return h->tune_cpl <= 1 && !(h->periodic || is_wide(h) || h->has_closure || h->turbulence) ? mode : 0;
 }
 
@@ -1739,9 +1743,12 @@ struct aslhip_batch {
 namespace {
 
 // Tiles for the batch's steps: non-periodic batches under 4 GiB per lattice;
-// automatic for FP32 storage, as for single lattices.
+// automatic as for single lattices (FP32 storage, the batch's populations
+// beyond the 256 MiB Infinity Cache).
 bool batch_tile(const aslhip_batch *b) {
-  const bool on = b->tune_lds < 0 ? b->dtype == ASLHIP_F32 : b->tune_lds != 0;
+  long long bytes = 0;
+  for (const aslhip_lattice *h : b->lat) bytes += h->qs_bytes * h->Q;
+  const bool on = b->tune_lds < 0 ? b->dtype == ASLHIP_F32 && bytes > TILE_MIN_BYTES : b->tune_lds != 0;
   return on && !b->periodic && !b->wide;
 }
 

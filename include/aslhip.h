@@ -382,8 +382,8 @@ int aslhip_stats(aslhip_lattice *h, int64_t out[4]);
  * consecutive cells through LDS (16-byte loads and stores of each
  * direction's slots for FP32, workgroup 256); 2 = tiles on even steps only,
  * odd steps direct; 0 = direct, one cell per lane (or the runs
- * ASLHIP_TUNE_CPL asks for).  Default: 1 for FP32 storage, 0 for FP16 /
- * BF16 / FP64 storage.  Tiles apply to the same lattices as runs of cells
+ * ASLHIP_TUNE_CPL asks for).  Default: 1 for FP32 storage over 256 MiB of
+ * populations (lattices the 256 MiB Infinity Cache cannot hold), else 0.  Tiles apply to the same lattices as runs of cells
  * when ASLHIP_TUNE_CPL is 0 or 1; results are bit for bit the same for
  * every value. */
 #define ASLHIP_TUNE_LDS 6
@@ -476,7 +476,8 @@ int aslhip_batch_step(aslhip_batch *b, int nsteps, unsigned flags);
 /* As aslhip_batch_step, timed with HIP events on the batch's stream. */
 int aslhip_batch_time_steps(aslhip_batch *b, int nsteps, unsigned flags, double *ms_total);
 /* Workgroup size (ASLHIP_TUNE_BLOCK), LDS-staged tiles (ASLHIP_TUNE_LDS 1
- * or 0; default: tiles for FP32 storage on non-periodic batches) or group
+ * or 0; default: tiles for non-periodic FP32 batches over 256 MiB of
+ * populations) or group
  * size (ASLHIP_TUNE_GROUP_MB) of the batch's launches; the autotuners choose
  * between tiles and the direct kernel's workgroup size (256/512/1024,
  * timing nsteps real eager steps each; returns 256 for tiles) and the group

@@ -150,8 +150,8 @@ def test_runs_under_graph_replay_and_tuning_changes():
 def test_run_length_follows_nz():
     for nz, want in ((12, 4), (10, 2), (9, 1)):
         L = asl_amd.Lattice((6, 5, nz), 19, nu=0.1)
-        # the default for FP32: LDS-staged tiles on every step, one cell per lane
-        assert L.tuning_value(_lib.TUNE_CPL) == 1 and L.tuning_value(_lib.TUNE_LDS) == 1
+        # the default for a small FP32 lattice: direct, one cell per lane
+        assert L.tuning_value(_lib.TUNE_CPL) == 1 and L.tuning_value(_lib.TUNE_LDS) == 0
         L.set_tuning(_lib.TUNE_CPL, 4)
         assert L.tuning_value(_lib.TUNE_CPL) == want
         L.set_tuning(_lib.TUNE_CPL, 2)
@@ -169,15 +169,18 @@ def test_run_length_follows_nz():
     L.set_tuning(_lib.TUNE_CPL, 4)  # runs take precedence over tiles
     assert L.tuning_value(_lib.TUNE_LDS) == 0 and L.tuning_value(_lib.TUNE_CPL) == 4
     L.set_tuning(_lib.TUNE_CPL, 0)
-    assert L.tuning_value(_lib.TUNE_LDS) == 1 and L.tuning_value(_lib.TUNE_CPL) == 1
-    assert L.tuning() == (256, 4)
+    assert L.tuning_value(_lib.TUNE_LDS) == 0 and L.tuning_value(_lib.TUNE_CPL) == 1
+    L.set_tuning(_lib.TUNE_LDS, 1)
+    assert L.tuning_value(_lib.TUNE_LDS) == 1 and L.tuning() == (256, 4)
     L.set_tuning(_lib.TUNE_LDS, 2)
     assert L.tuning() == (512, 4)
     L.set_tuning(_lib.TUNE_LDS, 0)
     assert L.tuning() == (512, 4)
     with pytest.raises(asl_amd.ASLError):
         L.set_tuning(_lib.TUNE_LDS, 3)
-    for dtype in ("f16", "bf16", "f64"):  # direct by default for these storage types
-        L = asl_amd.Lattice((6, 5, 12), 19, dtype=dtype, nu=0.1)
-        assert L.tuning_value(_lib.TUNE_LDS) == 0
-        assert L.tuning() == (512, 4 if dtype == "f64" else 64)  # 16-bit storage: XCD runs of 64
+    # automatic tiles: FP32 lattices over 256 MiB of populations (19 x 4 B x 3.7 M cells)
+    for dtype, want in (("f32", 1), ("f16", 0), ("bf16", 0), ("f64", 0)):
+        L = asl_amd.Lattice((160, 150, 150), 19, dtype=dtype, nu=0.1)
+        assert L.tuning_value(_lib.TUNE_LDS) == want, dtype
+        assert L.tuning() == ((256, 4) if want else (512, 4 if dtype in ("f32", "f64") else 64))
+        del L

@@ -97,7 +97,7 @@ def test_launch_tuning_and_autotune_keep_results():
         L.step(50)
         same(L, ref)
     L, _ = cavity()
-    assert L.tuning() == (256, 4)  # the defaults: LDS-staged tiles of 256 cells on every step
+    assert L.tuning() == (512, 4)  # the defaults for a small lattice: direct, workgroups of 512
     # candidates: tiles on every step x 6 XCD runs; tiles on even steps x 6 XCD
     # runs x 3 odd-step workgroups; direct with cells per lane 1 and 2 (nz = 22
     # is not a multiple of 4) x 6 XCD runs x 3 workgroup sizes; 10 real steps each
