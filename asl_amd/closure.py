@@ -283,11 +283,20 @@ def _ke_error(ke, ke_ref):
     return float(np.abs(ke / ke_ref - 1.0).max())
 
 
+def _shifted(params, shift):
+    """The network with its output bias b3 moved by shift (a uniform change of
+    y, i.e. of the relaxation-time correction by shift x output scale)."""
+    p = np.array(params, dtype=np.float32, copy=True)
+    p[-1] += np.float32(shift)
+    return p
+
+
 def study(n_coarse, tau_fine, chunk=50, snapshots=16, eval_chunks=16, target="total", seed=20261202,
-          train_steps=4000, batch=16384, grid=SCALE_GRID, ce_init=False, log=print):
+          train_steps=4000, batch=16384, grid=SCALE_GRID, shifts=(0.0,), ce_init=False, log=print):
     """Coarse n^3 with and without the learned closure against the 4x finer run.
 
-    The network is trained, and its output scale chosen from `grid`, on one
+    The network is trained, and its output scale chosen from `grid` (with
+    its output bias moved by one of `shifts`), on one
     realisation of the initial field (seed + 1): against the fine run's
     kinetic-energy decay (target "total") or the box-filtered fine kinetic
     energy ("filtered"), over the evaluation horizon of eval_chunks chunks of
@@ -314,14 +323,16 @@ def study(n_coarse, tau_fine, chunk=50, snapshots=16, eval_chunks=16, target="to
         ke_tr = tr.filtered_ke_history(chunk, eval_chunks)
     calib = {}
     for a in grid:
-        C = tr.coarse()
-        if a > 0:
-            C.set_closure(params, mu, sc, out * a)
-        ke_a, _ = tr.ke_history(C, chunk, eval_chunks)
-        calib[a] = _ke_decay_error(ke_a, ke_tr) if target == "total" else _ke_error(ke_a, ke_tr)
-        del C
-    alpha = min(calib, key=calib.get)
-    log("# output scale on the training field (%s kinetic energy): %s -> %s" % (target, calib, alpha))
+        for sh in (shifts if a > 0 else (0.0,)):
+            C = tr.coarse()
+            if a > 0:
+                C.set_closure(_shifted(params, sh), mu, sc, out * a)
+            ke_a, _ = tr.ke_history(C, chunk, eval_chunks)
+            calib[(a, sh)] = _ke_decay_error(ke_a, ke_tr) if target == "total" else _ke_error(ke_a, ke_tr)
+            del C
+    alpha, shift = min(calib, key=calib.get)
+    log("# output scale, bias shift on the training field (%s kinetic energy): %s -> %s, %s"
+        % (target, calib, alpha, shift))
     del tr
     ex = Experiment(n_coarse=n_coarse, tau_fine=tau_fine, seed=seed, ce_init=ce_init)
     F = ex.fine()
@@ -334,7 +345,7 @@ def study(n_coarse, tau_fine, chunk=50, snapshots=16, eval_chunks=16, target="to
     ke_p, wall_p = ex.ke_history(P, chunk, eval_chunks)
     del P
     C = ex.coarse()
-    C.set_closure(params, mu, sc, out * max(alpha, 1e-30))
+    C.set_closure(_shifted(params, shift), mu, sc, out * max(alpha, 1e-30))
     ke_c, wall_c = ex.ke_history(C, chunk, eval_chunks)
     del C
     return {"study": "learned closure, D3Q19 decaying turbulence", "coarse": n_coarse, "fine": ex.nf,
@@ -342,7 +353,8 @@ def study(n_coarse, tau_fine, chunk=50, snapshots=16, eval_chunks=16, target="to
             "start": {False: "equilibrium", True: "equilibrium + Chapman-Enskog term",
                       "coarse": "coarse: equilibrium + Chapman-Enskog term; fine: equilibrium"}[ce_init],
             "coarse_steps_per_chunk": chunk, "train_snapshots": snapshots, "eval_chunks": eval_chunks,
-            "output_scale_factor": alpha, "calibration_errors": {str(k): v for k, v in calib.items()},
+            "output_scale_factor": alpha, "output_bias_shift": shift,
+            "calibration_errors": {"%s/%s" % k: v for k, v in calib.items()},
             "ke_fine": ke_f.tolist(), "ke_fine_filtered": ke_ff.tolist(), "ke_coarse_plain": ke_p.tolist(),
             "ke_coarse_closure": ke_c.tolist(),
             "max_rel_ke_decay_error_plain": _ke_decay_error(ke_p, ke_f),
