@@ -46,6 +46,7 @@ SIGNATURES = {
     "aslhip_set_compressible": [_vp, ctypes.c_int],
     "aslhip_set_omega": [_vp, _dp],
     "aslhip_set_geometry": [_vp, ctypes.c_double, _dp],
+    "aslhip_set_vmap_skip_outer": [_vp, ctypes.c_int],
     "aslhip_macro_device": [_vp, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
     "aslhip_set_turbulence": [_vp, ctypes.c_int],
     "aslhip_set_compute_macro": [_vp, ctypes.c_int, ctypes.c_int],

@@ -1,4 +1,5 @@
 // asl::LBGK and asl::LBGKUtilities over the C ABI (include/aslhip.h).
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -127,6 +128,9 @@ void LBGK::create(const Block &b, acl::TypeID t) {
   // positions of the rotating velocity-map BCs: the block's dx and position
   const double pos[3] = {b.position[0], b.position[1], b.position[2]};
   check_abi(aslhip_set_geometry(h_, b.dx, pos));
+  // opt-in defined value for velocity-map averages over outer-layer cells
+  if (const char *e = std::getenv("ASLHIP_VMAP_SKIP_OUTER"))
+    if (std::string(e) == "1") check_abi(aslhip_set_vmap_skip_outer(h_, 1));
   rho_ = std::make_shared<DataWithGhostNodesACLData>(b, 1, t);
   vel_ = std::make_shared<DataWithGhostNodesACLData>(b, 3, t);
   f_ = std::make_shared<DataWithGhostNodesACLData>(b, q, t);

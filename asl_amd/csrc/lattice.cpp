@@ -189,6 +189,7 @@ struct aslhip_lattice {
   int tune_remap = 4;  // chunk runs per XCD (measured best for graph chunks; 64 for 16-bit storage; the autotuner may choose others)
   int tune_cpl = 0;    // cells per lane along z: 0 automatic, else 1, 2 or 4 (ASLHIP_TUNE_CPL)
   int tune_lds = -1;   // LDS-staged tiles: 0 off, 1 every step, 2 even steps, -1 automatic (ASLHIP_TUNE_LDS)
+  bool vmap_skip_outer = false;  // velocity-map averages skip outer-layer neighbours (opt-in)
   // face rule: per face slot * 4 + mode of the winning slice BC, and values
   int fface[6] = {0, 0, 0, 0, 0, 0};
   float fval32[6][27] = {};
@@ -541,6 +542,10 @@ int finalize_impl(aslhip_lattice *h, bool allow_open = false) {
         for (int i = 1; i < Q; ++i) {
           if (!((l.comp >> i) & 1u)) continue;
           const int X = gx + t.c[i][0], Y = gy + t.c[i][1], Z = gz + t.c[i][2];
+          if (!interior(X, Y, Z) && h->vmap_skip_outer) {
+            l.comp &= ~(1u << i);  // left out of the density average (opt-in)
+            continue;
+          }
           if (!interior(X, Y, Z) || !computed[h->idx(X, Y, Z)])
             return fail(ASLHIP_EUNDEF,
                         "ASL ERROR: velocity map BC at ghost (%d,%d,%d) averages a neighbour that is not a "
@@ -1049,6 +1054,14 @@ int aslhip_set_omega(aslhip_lattice *h, const double omega[3]) {
   if (int r = flush_deferred(h)) return r;  // the pending step runs with the old frame
   h->has_omega = omega != nullptr;
   for (int d = 0; d < 3; ++d) h->omega[d] = omega ? omega[d] : 0.;
+  return 0;
+}
+
+int aslhip_set_vmap_skip_outer(aslhip_lattice *h, int on) {
+  if (int r = check(h)) return r;
+  if (int r = flush_deferred(h)) return r;  // a pending step uses the current link tables
+  if (h->vmap_skip_outer != (on != 0)) h->dirty = true;
+  h->vmap_skip_outer = on != 0;
   return 0;
 }
 

@@ -106,6 +106,12 @@ class Lattice:
         position, where rotating velocity-map BCs evaluate omega x (r - centre)."""
         call("aslhip_set_geometry", self.h, float(dx), (ctypes.c_double * 3)(*[float(x) for x in position]))
 
+    def set_vmap_skip_outer(self, on=True):
+        """Extension, an opt-in defined value: velocity-map density averages
+        leave out neighbours in the block's outer ghost layer (otherwise a
+        lattice that reads them is refused)."""
+        call("aslhip_set_vmap_skip_outer", self.h, int(bool(on)))
+
     def set_compute_macro(self, rho=True, vel=True):
         call("aslhip_set_compute_macro", self.h, int(bool(rho)), int(bool(vel)))
 

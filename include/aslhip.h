@@ -146,6 +146,16 @@ int aslhip_set_omega(aslhip_lattice *h, const double omega[3]);
  * src/math/aslIndex2Position.cxx:64-108).  Default: dx 1, position 0. */
 int aslhip_set_geometry(aslhip_lattice *h, double dx, const double position[3]);
 
+/* Extension (no counterpart in ASL), an opt-in defined value: velocity-map
+ * BCs (src/num/aslLBGKBC.cxx:290-348) average the density over the
+ * computation-node neighbours of a boundary node; ASL takes neighbours in
+ * the block's outer ghost layer too and reads whatever its pool memory holds
+ * there.  on = 1: such neighbours are left out of the average (and of its
+ * weight sum); on = 0 (default): a lattice that would read them is refused
+ * with ASLHIP_EUNDEF.  The asl:: layer sets 1 when the environment variable
+ * ASLHIP_VMAP_SKIP_OUTER is 1. */
+int aslhip_set_vmap_skip_outer(aslhip_lattice *h, int on);
+
 /* compRho / compVel constructor flags (src/num/aslLBGK.cxx:210-218). */
 int aslhip_set_compute_macro(aslhip_lattice *h, int comp_rho, int comp_vel);
 

@@ -52,6 +52,7 @@ def load_library():
     lib.orc_closure_features.restype = ctypes.c_int
     lib.orc_set_compute_macro.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     lib.orc_set_geometry.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
+    lib.orc_set_vmap_skip_outer.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_init_equilibrium.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
     lib.orc_init_equilibrium_field.argtypes = [ctypes.c_void_p, _dp, _dp]
     lib.orc_bc_add_slice.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_double, _dp]
@@ -223,6 +224,10 @@ class OracleLattice:
 
     def set_compute_macro(self, rho=True, vel=True):
         self.lib.orc_set_compute_macro(self.h, int(rho), int(vel))
+
+    def set_vmap_skip_outer(self, on=True):
+        """Extension: velocity-map density averages skip outer-ghost-layer neighbours."""
+        self.lib.orc_set_vmap_skip_outer(self.h, 1 if on else 0)
 
     def set_geometry(self, dx=1.0, position=(0.0, 0.0, 0.0)):
         self.lib.orc_set_geometry(self.h, float(dx), _d(_vec3(position)))

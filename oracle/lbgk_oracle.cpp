@@ -177,6 +177,9 @@ struct Lattice {
   // LBGKTurbulence (src/num/aslLBGK.cxx:298-355)
   bool turb = false;
   double dx = 1., pos0[3] = {0., 0., 0.};
+  // Extension: velocity-map density averages skip neighbours in the outer
+  // ghost layer (ASL reads them as whatever its pool memory holds there).
+  bool vmap_skip_outer = false;
   std::vector<BC<T>> bcs;
   // per-direction constants as ASL prints them (double -> T)
   T cw[27];       // w_i
@@ -580,6 +583,13 @@ struct Lattice {
     }
   }
 
+  // idx in the block's outer ghost layer (or outside the block)
+  bool outer_layer(long idx) const {
+    if (idx < 0 || idx >= len) return true;
+    const long x = idx / Tr[0], y = (idx % Tr[0]) / Tr[1], z = (idx % Tr[0]) % Tr[1];
+    return x == 0 || y == 0 || z == 0 || x == S[0] - 1 || y == S[1] - 1 || z == S[2] - 1;
+  }
+
   bool comp_node(const BC<T> &bc, long idx) const {
     // isComputationNode (src/num/aslBCond.cxx:204-211;
     // src/math/aslDistanceFunctionAlg.cxx:45-48): map > 0 [&& cdomain > 0]
@@ -764,6 +774,7 @@ struct Lattice {
           T coef = T(0), r = T(0);
           for (int i = 1; i < Q; ++i) {
             if (!comp_node(bc, j + off(i))) continue;
+            if (vmap_skip_outer && outer_layer(j + off(i))) continue;
             coef = coef + cw[i];
             r = r + f_safe(vt.inv[i], j + off(i)) * cw[i];
           }
@@ -987,6 +998,9 @@ void orc_set_compute_macro(void *h, int r, int v) {
     L.comp_rho = r != 0;
     L.comp_vel = v != 0;
   });
+}
+void orc_set_vmap_skip_outer(void *h, int on) {
+  dispatch(h, [&](auto &L) { L.vmap_skip_outer = on != 0; });
 }
 void orc_set_geometry(void *h, double dx, const double pos[3]) {
   dispatch(h, [&](auto &L) {

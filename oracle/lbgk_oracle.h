@@ -51,6 +51,8 @@ void orc_set_closure_mode(void *h, int mode);
 int orc_closure_features(void *h, float *out);
 void orc_set_compute_macro(void *h, int comp_rho, int comp_vel);
 void orc_set_geometry(void *h, double dx, const double pos[3]);
+/* Extension: velocity-map density averages skip outer-ghost-layer neighbours. */
+void orc_set_vmap_skip_outer(void *h, int on);
 void orc_init_equilibrium(void *h, double rho, const double u[3]);
 void orc_init_equilibrium_field(void *h, const double *rho, const double *u);
 int orc_bc_add_slice(void *h, int kind, unsigned faces, double p, const double u[3]);

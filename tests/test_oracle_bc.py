@@ -168,3 +168,52 @@ def test_velocity_map_matches_link_model(rotating):
     O, M, comp = _run_both(size, 19, 0.05, 10, setup, lambda ids: list(ids))
     _compare(O, M, comp)
     assert np.abs(O.velocity()[:, comp]).max() > 1e-3
+
+
+def _casing(size, radius):
+    """Fluid inside a z-cylinder, map = (radius - r) / 3 clipped to [-1, 1]: a
+    guard band three cells thick, so no cell beyond it neighbours the fluid."""
+    S = tuple(n + 2 for n in size)
+    x = np.arange(S[0], dtype=np.float64)[:, None, None] - (S[0] - 1) / 2.0
+    y = np.arange(S[1], dtype=np.float64)[None, :, None] - (S[1] - 1) / 2.0
+    m = (radius - np.sqrt(x * x + y * y)) / 3.0 + np.zeros((1, 1, S[2]))
+    return np.clip(m, -1.0, 1.0).astype(np.float32)
+
+
+def test_velocity_map_skip_outer_option():
+    # Extension (opt-in defined value): velocity-map density averages leave
+    # out neighbours in the outer ghost layer.  On a casing whose boundary
+    # nodes in the first and last z layer have such neighbours (ASL reads the
+    # values the Z0 / ZE pressure BC left there) the option changes the run;
+    # where no outer neighbour is involved it changes nothing, bit for bit.
+    size = (14, 14, 6)
+    m = _casing(size, 4.5)
+    inner = (slice(1, -1),) * 3
+    fluid = m[inner] > 0
+    res = []
+    for skip in (False, True):
+        O = OracleLattice(size, 15, fp64=False, nu=0.05)
+        O.set_vmap_skip_outer(skip)
+        O.init_f(1.0, (0.0, 0.0, 0.0))
+        ids = [O.add_map_bc(VELOCITY_MAP, m, u=(0.0, 0.0, 0.0), rot=(0.0, 0.0, 0.01, 7.5, 7.5, 3.5)),
+               O.add_slice_bc(CONSTANT_PRESSURE, 16 | 32, p=1.0)]
+        for b in ids:
+            O.execute_bc(b)
+        O.run(6, ids)
+        r = O.rho()[inner][fluid]
+        assert np.isfinite(r).all()
+        res.append(r)
+    assert not np.array_equal(res[0], res[1])
+    size = (9, 8, 8)
+    body = geometry.normalize(geometry.sphere(size, 2.2, (4.0, 3.5, 3.5)))
+    out = []
+    for skip in (False, True):
+        O = OracleLattice(size, 19, fp64=False, nu=0.05)
+        O.set_vmap_skip_outer(skip)
+        O.init_f(1.0, (0.0, 0.0, 0.0))
+        ids = [O.add_slice_bc(NOSLIP, ALL), O.add_map_bc(VELOCITY_MAP, body, u=(0.02, -0.01, 0.01))]
+        for b in ids:
+            O.execute_bc(b)
+        O.run(10, ids)
+        out.append(O.f())
+    assert np.array_equal(out[0], out[1], equal_nan=True)
