@@ -487,9 +487,57 @@ def test_asl_compressor_driver_with_outer_average_opt_in(tmp_path):
                        env=env)
     assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
     assert "Finished" in p.stdout
-    out = read_vti(tmp_path / "compressor_5.vti")  # after 10,000 steps
-    fluid = (out["case"] > 0) & (out["compressor"] > 0)
-    assert fluid.sum() > 100000 and (out["compressor"] <= 0).sum() > 1000
-    assert np.isfinite(out["rho"][fluid]).all() and np.isfinite(out["v"][:, fluid]).all()
-    assert np.abs(out["v"][:, fluid]).max() > 1e-4
+    last = read_vti(tmp_path / "compressor_5.vti")  # after 10,000 steps
+    fluid = (last["case"] > 0) & (last["compressor"] > 0)
+    assert fluid.sum() > 100000 and (last["compressor"] <= 0).sum() > 1000
+    assert np.isfinite(last["rho"][fluid]).all() and np.isfinite(last["v"][:, fluid]).all()
+    # the output after 2000 steps against the oracle run of the same set-up
+    # (with the same opt-in): maps, rho on the fluid nodes, u on the fluid
+    # and the visualisation-filled boundary nodes, bit for bit
+    import oracle
+    from asl_amd import geometry
+    from asl_driver_geometry import block_bpos, cylinder_inf
+    out = read_vti(tmp_path / "compressor_1.vti")
+    dx = 0.5
+    grid = [int(v * (1.0 / dx)) for v in (33.0, 33.0, 90.0)]       # the Block given to readSurface
+    gpos = [-16.65, -26.15, -67.7]
+    size = tuple(n - 2 for n in grid)                               # the lattice's internal block
+    ipos = [v + dx for v in gpos]
+    tri = cube_triangles(np.array([-55.0, -14.0, -5.0]), np.array([-35.0, -5.0, 5.0]))
+    t_asl = tri.astype(np.float32).astype(np.float64)[..., ::-1]
+    h = [((gpos[d] + (grid[d] - 1) * dx) - gpos[d]) / (grid[d] - 1) for d in range(3)]
+    body = oracle.surface_distance_map(t_asl, grid, gpos, h, 3.6 * dx, 1.2 * dx, dx, 1.8)
+    bp = block_bpos(ipos, size, dx)
+    r_case = (bp[1] - ipos[1]) / 2.
+    centre = [.5 * (bp[d] + ipos[d]) for d in range(3)]
+    centre[1] = ipos[1] + r_case
+    case = oracle.df_map(geometry.df_normalize(geometry.df_neg(cylinder_inf(r_case, [0., 0., 1.], centre)), dx),
+                         grid, gpos, dx)
+    inner = (slice(1, -1),) * 3
+    assert np.array_equal(body[inner], out["compressor"]) and np.array_equal(case[inner], out["case"])
+    w_num = 6. * 3.14 * 2. / 60. * .001
+    rot = (0., 0., w_num / dx) + tuple(.5 * (bp[d] + ipos[d]) for d in range(3))
+    O = oracle.OracleLattice(size, 15, fp64=False, nu=float(np.float32(0.2)))
+    O.set_geometry(dx, ipos)
+    O.set_omega((0., 0., w_num))
+    O.set_vmap_skip_outer(True)
+    O.init_f(1.0, (0.0, 0.0, 0.0))
+    ids = [O.add_map_bc(O.VELOCITY_MAP, case, u=(0., 0., 0.), rot=rot),
+           O.add_slice_bc(O.CONSTANT_PRESSURE, 16 | 32, p=1.0), O.add_map_bc(O.NOSLIP_MAP, body)]
+
+    def vis():
+        O.vis_fill(3, case, u=(0., 0., 0.), rot=rot)
+        O.vis_fill(1, body)
+    for b in ids:
+        O.execute_bc(b)
+    vis()
+    O.run(2000, ids)
+    vis()
+    guard = lambda m: (m[inner] <= 0) & (m[inner] > -0.9999)  # noqa: E731
+    fluid = (case[inner] > 0) & (body[inner] > 0)
+    keep = fluid | (guard(case) & (body[inner] > 0)) | (guard(body) & (case[inner] > 0))
+    assert np.array_equal(out["rho"][fluid], O.rho()[inner].astype(np.float32)[fluid])
+    v_o = O.velocity()[(slice(None),) + inner].astype(np.float32)
+    assert np.array_equal(out["v"][:, keep], v_o[:, keep])
+    assert np.abs(v_o[:, fluid]).max() > 1e-4
 
