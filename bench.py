@@ -472,12 +472,12 @@ def batch_study(steps, warmup, group_mb=None):
                       "per_lattice_interleaved_mlups": round(cells * steps / ms_interleaved / 1e3, 1)}), flush=True)
 
 
-def closure_study(nc, tau_fine, chunk, snaps, eval_chunks, target="total", ce_init=False, shifts=(0.0,)):
+def closure_study(nc, tau_fine, chunk, snaps, eval_chunks, target="total", ce_init=False, shifts=(0.0,), seed=SEED):
     """Coarse N^3 with and without the learned closure against the 4x finer run
     (``asl_amd.closure.study``): one JSON line."""
     from asl_amd import closure
     r = closure.study(nc, tau_fine, chunk=chunk, snapshots=snaps, eval_chunks=eval_chunks, target=target,
-                      seed=SEED, ce_init=ce_init, shifts=shifts, log=lambda m: print(m, flush=True))
+                      seed=seed, ce_init=ce_init, shifts=shifts, log=lambda m: print(m, flush=True))
     print(json.dumps(r), flush=True)
 
 
@@ -560,7 +560,9 @@ def main():
     ap.add_argument("--ce-start", choices=("none", "coarse", "both"), default="none",
                     help="closure study: start the coarse lattice (or both) from equilibrium plus the first "
                          "Chapman-Enskog term of the field instead of equilibrium")
-    ap.add_argument("--closure-shifts", default="-0.6,-0.3,0,0.3",
+    ap.add_argument("--closure-seed", type=int, default=SEED,
+                    help="closure study: seed of the evaluation field (the training field uses seed + 1)")
+    ap.add_argument("--closure-shifts", default="-1.2,-0.9,-0.6,-0.3,0,0.3",
                     help="closure study: output-bias shifts tried with each output scale (comma-separated)")
     ap.add_argument("--variant-study", type=int, default=0, metavar="N",
                     help="time plain / LBGKTurbulence / closure step kernels on a periodic N^3 lattice")
@@ -590,7 +592,7 @@ def main():
     if args.closure_study:
         closure_study(args.closure_study, args.tau_fine, args.chunk, args.train_snapshots, args.eval_chunks,
                       args.closure_target, {"none": False, "coarse": "coarse", "both": True}[args.ce_start],
-                      tuple(float(v) for v in args.closure_shifts.split(",")))
+                      tuple(float(v) for v in args.closure_shifts.split(",")), args.closure_seed)
         return
     if args.batch_study:
         batch_study(args.steps if args.steps != 5000 else 1000, args.warmup if args.warmup != 200 else 20,
